@@ -1,0 +1,219 @@
+"""Throughput benchmark of the fused OCT chain on one MI355X.
+
+Workload (default): 2048-sample A-scans, 1024 A-scans x 16 B-scans per
+frame, uint16 input; fixed-pattern-noise removal (one-shot determination
+during warm-up), cubic k-linearization against a custom resampling curve,
+Hann window (centre 0.5, fill 0.95), dispersion phase
+5e-6 (k-1024)^2 - 2e-9 (k-1024)^3, float32 inverse FFT, log scaling.
+Synthetic seeded fringes (3 reflectors per A-scan) are generated on the GPU.
+
+A step is one frame through ope_hip_process_device (device-resident input
+and output; a ring of 8 distinct frames, 1 GiB of input+output, so the
+256 MB last-level cache cannot hold the working set).  W warm-up steps, then
+K timed steps bracketed by device synchronisation, timed with HIP events on
+the stream the kernels run on.
+
+Printed: one JSON line with the throughput (A-scans/s), the roofline of the
+chain kernel (HBM-bound: algorithmic bytes = N*2 B in + N/2*4 B out per
+A-scan) and the CPU baseline (the float32 C restatement of the reference CPU
+backend in oracle/, timed on a bounded sample on this host).
+"""
+
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, AMD published
+
+
+def workload_c2():
+    return dict(workload="2048x1024x16 uint16, FPN + cubic + Hann + dispersion + log",
+                n=2048, ascans_per_bscan=1024, bscans=16, interpolation=1, fpn=True,
+                dispersion=True, log=True)
+
+
+def build_context(cfg, capi, S, window_fn, phasor_fn):
+    n = cfg["n"]
+    params = capi.make_params(
+        signal_length=n, ascans_per_bscan=cfg["ascans_per_bscan"], bscans_per_buffer=cfg["bscans"],
+        input_type=capi.UINT16, resampling_enabled=1, interpolation=cfg["interpolation"],
+        windowing_enabled=1, dispersion_enabled=int(cfg["dispersion"]), log_scaling=1,
+        grayscale_min=30.0, grayscale_max=100.0, fpn_removal=int(cfg["fpn"]), fpn_bscan_count=1)
+    ctx = capi.Context(params)
+    ctx.set_resampling_curve(S.resampling_curve(n))
+    ctx.set_window_curve(window_fn(n))
+    ctx.set_dispersion_curve(phasor_fn(S.dispersion_phase(n, 5e-6, -2e-9)))
+    if cfg["fpn"]:
+        ctx.request_fpn()
+    return ctx
+
+
+def hann_window(n):
+    """Hann window as generated by the reference (processorconfiguration.cpp:501-512)."""
+    import numpy as np
+    width = int(0.95 * n)
+    min_pos = int(0.5 * n) - width // 2
+    x = (np.arange(n) - min_pos).astype(np.float32) / np.float32(width - 1.0)
+    w = (0.5 * (1.0 - np.cos(2.0 * np.pi * x.astype(np.float64)))).astype(np.float32)
+    w[(x > np.float32(0.999)) | (x < np.float32(0.0001))] = 0.0
+    return w
+
+
+def cos_sin(phase):
+    import numpy as np
+    out = np.empty(2 * phase.size, dtype=np.float32)
+    out[0::2] = np.cos(phase)
+    out[1::2] = np.sin(phase)
+    return out
+
+
+def cpu_baseline(cfg, seconds: float, threads: int):
+    """The oracle's float32 C restatement of the reference CPU backend on
+    frames of the same workload, single worker thread like the reference
+    (cpu_backend.cpp:428-431), for about `seconds` of wall clock."""
+    import numpy as np
+
+    from octproengine_amd import synthetic as S
+    from tests.helpers import CpuParams, _fp, cpu_lib
+    n = cfg["n"]
+    a = cfg["ascans_per_bscan"]  # one B-scan per sample frame
+    raw = S.fringes(n, a, seed=0)
+    curve = S.resampling_curve(n)
+    win = hann_window(n)
+    ph = cos_sin(S.dispersion_phase(n, 5e-6, -2e-9))
+    fpn = np.zeros(n, dtype=np.float32)
+    lib = cpu_lib()
+    P = CpuParams(n, a, 2, 1, cfg["interpolation"], 1, int(cfg["dispersion"]), 0, 64, 1, 30.0, 100.0,
+                  1.0, 0.0, n // 2, 2 * n)
+    out = np.zeros((a, n // 2), dtype=np.float32)
+    done, t0 = 0, time.perf_counter()
+    while True:
+        lib.ope_cpu_process(ctypes.byref(P), raw.ctypes.data, _fp(curve), _fp(win), _fp(ph),
+                            fpn.ctypes.data, out.ctypes.data, threads)
+        done += a
+        dt = time.perf_counter() - t0
+        if dt >= seconds:
+            break
+    return done / dt, done, dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--ring", type=int, default=8, help="distinct frames cycled through")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(A-scans are independent; see the sharding notes in DESIGN.md).", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from octproengine_amd import _capi as capi
+    from octproengine_amd import synthetic as S
+
+    cfg = workload_c2()
+    n = cfg["n"]
+    frame = cfg["ascans_per_bscan"] * cfg["bscans"]
+    dev = torch.device("cuda:0")
+    print(f"[bench] generating {args.ring} frames of {frame} A-scans x {n} samples on {torch.cuda.get_device_name(0)}",
+          file=sys.stderr, flush=True)
+    ins = [S.fringes_torch(n, frame, seed=i, device="cuda") for i in range(args.ring)]
+    outs = [torch.empty((frame, n // 2), dtype=torch.float32, device=dev) for _ in range(args.ring)]
+    ctx = build_context(cfg, capi, S, hann_window, cos_sin)
+    stream = torch.cuda.current_stream()
+    sp = stream.cuda_stream
+
+    def step(i):
+        ctx.process_device(ins[i % args.ring].data_ptr(), outs[i % args.ring].data_ptr(), frame, sp)
+
+    for i in range(max(args.warmup, 2)):  # >= 2: FPN determined + re-accumulated
+        step(i)
+    torch.cuda.synchronize()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ev0.record(stream)
+    for i in range(args.steps):
+        step(i)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    gpu_ms = ev0.elapsed_time(ev1)
+    ms_per_step = gpu_ms / args.steps
+    ascans_per_s = frame * args.steps / (gpu_ms / 1e3)
+
+    # batched form: the ring's frames in one launch (same kernel, one grid)
+    big_in = torch.cat(ins[:4], dim=0)
+    big_out = torch.empty((4 * frame, n // 2), dtype=torch.float32, device=dev)
+    ctx.process_device(big_in.data_ptr(), big_out.data_ptr(), 4 * frame, sp)
+    torch.cuda.synchronize()
+    reps = max(3, args.steps // 8)
+    ev0.record(stream)
+    for _ in range(reps):
+        ctx.process_device(big_in.data_ptr(), big_out.data_ptr(), 4 * frame, sp)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    batched = 4 * frame * reps / (ev0.elapsed_time(ev1) / 1e3)
+
+    # sanity: the last output is finite and a plausible grayscale image
+    o = outs[(args.steps - 1) % args.ring]
+    assert torch.isfinite(o).all().item()
+
+    bytes_per_ascan = n * 2 + (n // 2) * 4
+    table_bytes = 24 * n
+    algo_bytes = frame * bytes_per_ascan + table_bytes
+    achieved = algo_bytes / (ms_per_step / 1e3) / 1e9
+
+    result = {
+        "metric": "A-scans/sec (2048-pt full chain) on 1 MI355X + % HBM roofline vs host CPU",
+        "value": round(ascans_per_s, 1),
+        "unit": "A-scans/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded fringes generated on the GPU)",
+        "config": {"workload": cfg["workload"], "signal_length": n,
+                   "ascans_per_frame": frame, "input": "uint16", "frames_in_ring": args.ring,
+                   "wall_s_timed_region": round(wall, 4)},
+        "batched_4_frames_per_launch": {"value": round(batched, 1), "unit": "A-scans/s"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "chain_kernel<2048, 4 taps>", "bytes_per_launch": algo_bytes},
+    }
+    if not args.no_cpu_baseline:
+        v1, done1, dt1 = cpu_baseline(cfg, args.cpu_seconds, 1)
+        result["cpu_baseline"] = {
+            "value": round(v1, 1), "unit": "A-scans/s", "cores": 1, "kind": "port",
+            "sample": f"{done1} A-scans ({done1 // cfg['ascans_per_bscan']} B-scans of 1024) of the same "
+                      f"chain in {dt1:.1f} s, 1 worker thread as the reference; float32 C restatement "
+                      f"with a self-written radix-4 FFT (FFTW absent); host nproc={os.cpu_count()}",
+        }
+        result["gpu_over_cpu"] = round(ascans_per_s / v1, 1)
+    print(json.dumps(result), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,204 @@
+/*
+ * ope_hip.h -- C ABI of the MI355X (gfx950) processing library
+ * (libope_hip.so).  This is the drop-in boundary behind which the HIP
+ * backend replaces OCTproEngine's CUDA backend: the C++ class
+ * ope::HipBackend (an implementation of the reference's
+ * ProcessingBackend, src/backends/backend_interface.h:12-165) calls only
+ * these functions, and the Python tests call them through ctypes.
+ *
+ * Conventions
+ *  - Every function returns an int status: OPE_HIP_OK (0) or a negative
+ *    OPE_HIP_E* code.  ope_hip_last_error(ctx) returns the message of the
+ *    last failure on that context (thread-unsafe by design, like errno on one
+ *    thread).  No C++ exception crosses this boundary.
+ *  - Plain pointers and sizes only.  "device" pointers are HIP device
+ *    memory; "host" pointers are ordinary or pinned host memory.
+ *  - One context drives one GPU (the device id given at creation).
+ *
+ * Each declaration names the reference interface it replaces.
+ */
+#ifndef OPE_HIP_H
+#define OPE_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#if defined(__GNUC__) || defined(__clang__)
+#define OPE_HIP_API __attribute__((visibility("default")))
+#else
+#define OPE_HIP_API
+#endif
+
+/* Status codes. */
+#define OPE_HIP_OK 0
+#define OPE_HIP_EINVAL -1    /* bad argument (null pointer, wrong length)  */
+#define OPE_HIP_EUNSUP -2    /* configuration this build does not support  */
+#define OPE_HIP_ERUNTIME -3  /* HIP runtime error (message has the detail)  */
+#define OPE_HIP_ENOMEM -4    /* device or pinned-host allocation failed     */
+#define OPE_HIP_ESTATE -5    /* call not valid in the context's state       */
+
+/* Input sample types: the numeric values of ope::DataType
+ * (include/types.h:6-19).  UINT8/INT8 are read as 8-bit unsigned, UINT16/
+ * INT16 as 16-bit unsigned, UINT32/INT32 as 32-bit unsigned, exactly as the
+ * reference converts by bit depth (cpu_kernels.tpp:24-54).  FLOAT32 is read
+ * as IEEE float (the reference reinterprets its bits as uint32). */
+#define OPE_HIP_UINT8 1
+#define OPE_HIP_UINT16 2
+#define OPE_HIP_UINT32 3
+#define OPE_HIP_INT8 5
+#define OPE_HIP_INT16 6
+#define OPE_HIP_INT32 7
+#define OPE_HIP_FLOAT32 9
+
+/* Interpolation methods, ope::InterpolationMethod values
+ * (include/processorconfiguration.h:17-21). */
+#define OPE_HIP_INTERP_LINEAR 0
+#define OPE_HIP_INTERP_CUBIC 1
+#define OPE_HIP_INTERP_LANCZOS 2
+
+/* Plain-old-data snapshot of the ProcessorConfiguration fields the chain
+ * reads (include/processorconfiguration.h:33-172; read by
+ * CpuBackend::Impl::processData, cpu_backend.cpp:169-302).  Booleans are
+ * 0/1 ints. */
+typedef struct ope_hip_params {
+    int32_t signal_length;      /* N: power of two, 256..4096            */
+    int32_t ascans_per_bscan;
+    int32_t bscans_per_buffer;
+    int32_t input_type;         /* OPE_HIP_UINT16, ...                    */
+    int32_t resampling_enabled;
+    int32_t interpolation;      /* OPE_HIP_INTERP_*                       */
+    int32_t windowing_enabled;
+    int32_t dispersion_enabled;
+    int32_t dc_removal_enabled; /* rolling-average background removal     */
+    int32_t dc_window_size;
+    int32_t log_scaling;
+    float grayscale_min;
+    float grayscale_max;
+    float multiplicator;
+    float addend;
+    int32_t fpn_removal;        /* fixed-pattern-noise subtraction        */
+    int32_t fpn_bscan_count;
+    int32_t fpn_continuous;
+    int32_t bg_removal;         /* post-process background subtraction    */
+    float bg_weight;
+    float bg_offset;
+} ope_hip_params;
+
+typedef struct ope_hip_ctx ope_hip_ctx;
+
+/* Completion callback of a host frame: called on a HIP runtime thread after
+ * the output has landed in the host output buffer. */
+typedef void (*ope_hip_frame_cb)(void* user, float* host_out);
+
+/* Returns a static string naming the library build. */
+OPE_HIP_API const char* ope_hip_version(void);
+
+/* Number of visible HIP devices (0 when none). */
+OPE_HIP_API int ope_hip_device_count(int* count);
+
+/* Creates a context on `device` and allocates the device tables and
+ * `n_slots` (1..8) in-flight frame slots.
+ * Replaces CudaBackend::initialize (cuda_backend.cu:179-292). */
+OPE_HIP_API int ope_hip_create(int device, const ope_hip_params* params, int n_slots,
+                               ope_hip_ctx** out_ctx);
+
+/* Waits for in-flight work and frees every resource.
+ * Replaces CudaBackend::cleanup (cuda_backend.cu:294-385). */
+OPE_HIP_API int ope_hip_destroy(ope_hip_ctx* ctx);
+
+/* Message of the last failure on ctx ("" if none); ctx may be NULL for
+ * failures of ope_hip_create. */
+OPE_HIP_API const char* ope_hip_last_error(const ope_hip_ctx* ctx);
+
+/* Replaces the scalar/flag snapshot.  Dimensions (signal_length,
+ * ascans_per_bscan, bscans_per_buffer, input_type) must equal the ones the
+ * context was created with.  Replaces CudaBackend::updateConfig
+ * (cuda_backend.cu:776-782). */
+OPE_HIP_API int ope_hip_set_params(ope_hip_ctx* ctx, const ope_hip_params* params);
+
+/* Curve hot-swap.  Each call waits for in-flight frames, so a new curve
+ * never applies to half a frame.
+ *   resampling: N float positions (CpuBackend::updateResamplingCurve,
+ *               cpu_backend.cpp:485-490; CUDA cuda_backend.cu:784-797)
+ *   window:     N floats (cpu_backend.cpp:503-508; cuda_backend.cu:815-828)
+ *   dispersion: 2N floats, interleaved cos/sin (cpu_backend.cpp:492-501;
+ *               cuda_backend.cu:799-813).  Another length disables the
+ *               dispersion step, as the CPU backend does
+ *               (cpu_kernels.tpp:205-208). */
+OPE_HIP_API int ope_hip_set_resampling_curve(ope_hip_ctx* ctx, const float* curve, size_t length);
+OPE_HIP_API int ope_hip_set_window_curve(ope_hip_ctx* ctx, const float* curve, size_t length);
+OPE_HIP_API int ope_hip_set_dispersion_curve(ope_hip_ctx* ctx, const float* cos_sin,
+                                             size_t length);
+
+/* Fixed-pattern-noise profile, interleaved (re, im) pairs, k < N/2.
+ * set: CpuBackend::setFixedPatternNoiseProfile (cpu_backend.cpp:817-822);
+ *      pairs beyond N/2 are ignored, missing pairs subtract 0.  pairs == 0
+ *      clears the profile.
+ * get: writes min(pairs_capacity, stored) pairs; *pairs_out = stored pairs
+ *      (0 when no profile exists).  (cpu_backend.cpp:824-826)
+ * request: one-shot determination on the next frames
+ *      (cpu_backend.cpp:813-815, 305-357). */
+OPE_HIP_API int ope_hip_set_fpn_profile(ope_hip_ctx* ctx, const float* interleaved, size_t pairs);
+OPE_HIP_API int ope_hip_get_fpn_profile(ope_hip_ctx* ctx, float* interleaved, size_t pairs_capacity,
+                                        size_t* pairs_out);
+OPE_HIP_API int ope_hip_request_fpn(ope_hip_ctx* ctx);
+
+/* Post-process background profile, N/2 floats.
+ * set: CpuBackend::setPostProcessBackgroundProfile (cpu_backend.cpp:793-806)
+ *      (length must be N/2, else OPE_HIP_EINVAL)
+ * get: as for the FPN profile (cpu_backend.cpp:808-810)
+ * request: record from the next frame (cpu_backend.cpp:789-791,105-118). */
+OPE_HIP_API int ope_hip_set_bg_profile(ope_hip_ctx* ctx, const float* profile, size_t length);
+OPE_HIP_API int ope_hip_get_bg_profile(ope_hip_ctx* ctx, float* profile, size_t capacity,
+                                       size_t* length_out);
+OPE_HIP_API int ope_hip_request_bg_recording(ope_hip_ctx* ctx);
+
+/* Device-resident processing: runs the whole chain over `n_ascans` A-scans
+ * (a multiple of ascans_per_bscan is not required) from d_in (raw samples,
+ * input_type, [n_ascans][N]) to d_out (float, [n_ascans][N/2]) on `stream`
+ * (a hipStream_t; NULL = the HIP null stream).  Asynchronous.
+ * FPN accumulation/determination and background recording follow the
+ * frame semantics of CpuBackend::Impl::processData (cpu_backend.cpp:
+ * 169-357), treating the call as one frame.
+ * This is the GPU counterpart of the per-frame body of CudaBackend::process
+ * (cuda_backend.cu:486-743) without the host copies. */
+OPE_HIP_API int ope_hip_process_device(ope_hip_ctx* ctx, const void* d_in, float* d_out,
+                                       int64_t n_ascans, void* stream);
+
+/* Host frame: copies one frame (ascans_per_bscan * bscans_per_buffer
+ * A-scans) from host_in to the device, processes it and copies the result to
+ * host_out (N/2 floats per A-scan), all on the next free slot's stream.
+ * Returns once the work is enqueued.  `on_input_free(user_in)` (may be
+ * NULL) fires when host_in may be reused; `on_done(user, host_out)` fires
+ * after host_out is complete.  Replaces CudaBackend::process
+ * (cuda_backend.cu:442-773): H2D, chain, D2H, host callbacks. */
+OPE_HIP_API int ope_hip_submit_host(ope_hip_ctx* ctx, const void* host_in, float* host_out,
+                                    void (*on_input_free)(void* user_in), void* user_in,
+                                    ope_hip_frame_cb on_done, void* user);
+
+/* Blocks until all work submitted on the context has finished. */
+OPE_HIP_API int ope_hip_synchronize(ope_hip_ctx* ctx);
+
+/* Pinned host memory for input/output IOBuffers (hipHostMalloc; replaces
+ * the cudaHostRegister of cuda_backend.cu:220-233). */
+OPE_HIP_API int ope_hip_host_alloc(size_t bytes, void** ptr);
+OPE_HIP_API int ope_hip_host_free(void* ptr);
+
+/* Individual operation getMinimumVarianceMean (ProcessingBackend,
+ * backend_interface.h:117-122; CPU semantics cpu_backend.cpp:682-699 and
+ * cpu_kernels.tpp:402-458): for each of `width` columns of the interleaved
+ * complex rows input[height][width][2], the complex mean of the segment (of
+ * height/segments rows) with the smallest variance.  Host pointers; out has
+ * 2*width floats.  Synchronous. */
+OPE_HIP_API int ope_hip_op_min_variance_mean(ope_hip_ctx* ctx, const float* input, int width,
+                                             int height, int segments, float* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* OPE_HIP_H */

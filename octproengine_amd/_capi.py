@@ -1,0 +1,217 @@
+"""ctypes binding of the C ABI in include/ope_hip.h (libope_hip.so).
+
+The library is loaded from this package directory; there is no fallback:
+if it is missing or fails to load, importing this module raises.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+from ctypes import (POINTER, Structure, c_char_p, c_float, c_int, c_int32, c_int64, c_size_t,
+                    c_void_p)
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "libope_hip.so"
+
+OK, EINVAL, EUNSUP, ERUNTIME, ENOMEM, ESTATE = 0, -1, -2, -3, -4, -5
+
+# ope::DataType values (include/types.h)
+UINT8, UINT16, UINT32, INT8, INT16, INT32, FLOAT32 = 1, 2, 3, 5, 6, 7, 9
+DTYPE_CODES = {np.dtype(np.uint8): UINT8, np.dtype(np.uint16): UINT16,
+               np.dtype(np.uint32): UINT32, np.dtype(np.int8): INT8,
+               np.dtype(np.int16): INT16, np.dtype(np.int32): INT32,
+               np.dtype(np.float32): FLOAT32}
+LINEAR, CUBIC, LANCZOS = 0, 1, 2
+
+#: Every function include/ope_hip.h declares, with its ctypes signature.
+SIGNATURES = {
+    "ope_hip_version": (c_char_p, []),
+    "ope_hip_device_count": (c_int, [POINTER(c_int)]),
+    "ope_hip_create": (c_int, [c_int, c_void_p, c_int, POINTER(c_void_p)]),
+    "ope_hip_destroy": (c_int, [c_void_p]),
+    "ope_hip_last_error": (c_char_p, [c_void_p]),
+    "ope_hip_set_params": (c_int, [c_void_p, c_void_p]),
+    "ope_hip_set_resampling_curve": (c_int, [c_void_p, c_void_p, c_size_t]),
+    "ope_hip_set_window_curve": (c_int, [c_void_p, c_void_p, c_size_t]),
+    "ope_hip_set_dispersion_curve": (c_int, [c_void_p, c_void_p, c_size_t]),
+    "ope_hip_set_fpn_profile": (c_int, [c_void_p, c_void_p, c_size_t]),
+    "ope_hip_get_fpn_profile": (c_int, [c_void_p, c_void_p, c_size_t, POINTER(c_size_t)]),
+    "ope_hip_request_fpn": (c_int, [c_void_p]),
+    "ope_hip_set_bg_profile": (c_int, [c_void_p, c_void_p, c_size_t]),
+    "ope_hip_get_bg_profile": (c_int, [c_void_p, c_void_p, c_size_t, POINTER(c_size_t)]),
+    "ope_hip_request_bg_recording": (c_int, [c_void_p]),
+    "ope_hip_process_device": (c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_void_p]),
+    "ope_hip_submit_host": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                    c_void_p]),
+    "ope_hip_synchronize": (c_int, [c_void_p]),
+    "ope_hip_op_min_variance_mean": (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_void_p]),
+    "ope_hip_host_alloc": (c_int, [c_size_t, POINTER(c_void_p)]),
+    "ope_hip_host_free": (c_int, [c_void_p]),
+}
+
+
+class Params(Structure):
+    """ope_hip_params (include/ope_hip.h)."""
+    _fields_ = [
+        ("signal_length", c_int32), ("ascans_per_bscan", c_int32),
+        ("bscans_per_buffer", c_int32), ("input_type", c_int32),
+        ("resampling_enabled", c_int32), ("interpolation", c_int32),
+        ("windowing_enabled", c_int32), ("dispersion_enabled", c_int32),
+        ("dc_removal_enabled", c_int32), ("dc_window_size", c_int32),
+        ("log_scaling", c_int32), ("grayscale_min", c_float), ("grayscale_max", c_float),
+        ("multiplicator", c_float), ("addend", c_float), ("fpn_removal", c_int32),
+        ("fpn_bscan_count", c_int32), ("fpn_continuous", c_int32), ("bg_removal", c_int32),
+        ("bg_weight", c_float), ("bg_offset", c_float),
+    ]
+
+
+def make_params(**kw) -> Params:
+    """Params with the reference's defaults (processorconfiguration.h:33-172)."""
+    d = dict(signal_length=1024, ascans_per_bscan=512, bscans_per_buffer=1, input_type=UINT16,
+             resampling_enabled=0, interpolation=LINEAR, windowing_enabled=0,
+             dispersion_enabled=0, dc_removal_enabled=0, dc_window_size=64, log_scaling=1,
+             grayscale_min=30.0, grayscale_max=100.0, multiplicator=1.0, addend=0.0,
+             fpn_removal=0, fpn_bscan_count=1, fpn_continuous=0, bg_removal=0,
+             bg_weight=1.0, bg_offset=0.0)
+    unknown = set(kw) - set(d)
+    if unknown:
+        raise TypeError(f"unknown parameters: {sorted(unknown)}")
+    d.update(kw)
+    p = Params()
+    for k, v in d.items():
+        setattr(p, k, int(v) if isinstance(v, (bool, np.bool_)) else v)
+    return p
+
+
+def _load() -> ctypes.CDLL:
+    if not LIB_PATH.exists():
+        raise ImportError(f"{LIB_PATH} is not built; run `python -m octproengine_amd.build`")
+    lib = ctypes.CDLL(str(LIB_PATH), mode=os.RTLD_NOW | ctypes.RTLD_GLOBAL)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+lib = _load()
+
+
+class HipError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+def _f32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def device_count() -> int:
+    n = c_int(0)
+    lib.ope_hip_device_count(ctypes.byref(n))
+    return n.value
+
+
+class Context:
+    """One ope_hip_ctx: the device-side state of a HIP processing backend."""
+
+    def __init__(self, params: Params, device: int = 0, n_slots: int = 2):
+        self.params = params
+        h = c_void_p()
+        rc = lib.ope_hip_create(device, ctypes.byref(params), n_slots, ctypes.byref(h))
+        if rc != OK:
+            raise HipError(rc, lib.ope_hip_last_error(None).decode())
+        self._h = h
+
+    # -- plumbing
+    def _check(self, rc: int) -> None:
+        if rc != OK:
+            raise HipError(rc, lib.ope_hip_last_error(self._h).decode())
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib.ope_hip_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def handle(self) -> int:
+        return self._h.value
+
+    # -- configuration
+    def set_params(self, params: Params) -> None:
+        self._check(lib.ope_hip_set_params(self._h, ctypes.byref(params)))
+        self.params = params
+
+    def set_resampling_curve(self, curve) -> None:
+        c = _f32(curve)
+        self._check(lib.ope_hip_set_resampling_curve(self._h, c.ctypes.data, c.size))
+
+    def set_window_curve(self, curve) -> None:
+        c = _f32(curve)
+        self._check(lib.ope_hip_set_window_curve(self._h, c.ctypes.data, c.size))
+
+    def set_dispersion_curve(self, cos_sin) -> None:
+        c = _f32(cos_sin)
+        self._check(lib.ope_hip_set_dispersion_curve(self._h, c.ctypes.data, c.size))
+
+    def set_fpn_profile(self, interleaved) -> None:
+        c = _f32(interleaved)
+        self._check(lib.ope_hip_set_fpn_profile(self._h, c.ctypes.data, c.size // 2))
+
+    def get_fpn_profile(self) -> np.ndarray:
+        n = c_size_t(0)
+        half = self.params.signal_length // 2
+        buf = np.zeros(2 * half, dtype=np.float32)
+        self._check(lib.ope_hip_get_fpn_profile(self._h, buf.ctypes.data, half, ctypes.byref(n)))
+        return buf[: 2 * n.value]
+
+    def request_fpn(self) -> None:
+        self._check(lib.ope_hip_request_fpn(self._h))
+
+    def set_bg_profile(self, profile) -> None:
+        c = _f32(profile)
+        self._check(lib.ope_hip_set_bg_profile(self._h, c.ctypes.data, c.size))
+
+    def get_bg_profile(self) -> np.ndarray:
+        n = c_size_t(0)
+        half = self.params.signal_length // 2
+        buf = np.zeros(half, dtype=np.float32)
+        self._check(lib.ope_hip_get_bg_profile(self._h, buf.ctypes.data, half, ctypes.byref(n)))
+        return buf[: n.value]
+
+    def request_bg_recording(self) -> None:
+        self._check(lib.ope_hip_request_bg_recording(self._h))
+
+    # -- processing
+    def process_device(self, d_in: int, d_out: int, n_ascans: int, stream: int = 0) -> None:
+        """Device pointers (ints), asynchronous on `stream` (hipStream_t as int)."""
+        self._check(lib.ope_hip_process_device(self._h, c_void_p(d_in), c_void_p(d_out),
+                                               int(n_ascans), c_void_p(stream or None)))
+
+    def min_variance_mean(self, rows: np.ndarray, segments: int = 8) -> np.ndarray:
+        """rows: complex [height, width] -> interleaved float32 [2*width]."""
+        rows = np.asarray(rows)
+        h, w = rows.shape
+        inter = np.empty((h, w, 2), dtype=np.float32)
+        inter[..., 0] = rows.real
+        inter[..., 1] = rows.imag
+        out = np.zeros(2 * w, dtype=np.float32)
+        self._check(lib.ope_hip_op_min_variance_mean(self._h, inter.ctypes.data, w, h, segments,
+                                                     out.ctypes.data))
+        return out
+
+    def synchronize(self) -> None:
+        self._check(lib.ope_hip_synchronize(self._h))

@@ -1,0 +1,60 @@
+"""Builds the native parts of octproengine_amd in-tree.
+
+  libope_hip.so     HIP kernels + C ABI (include/ope_hip.h), gfx950 only
+  libope.so         C++ host library: ope::Processor, ProcessorConfiguration,
+                    IOBuffer, CallbackManager and the HIP backend
+  octproengine*.so  pybind11 module mirroring the reference's bindings
+
+Run ``python -m octproengine_amd.build`` (or ``__graft_entry__.build()``).
+Everything is compiled with hipcc/g++ directly; no CMake is needed.
+"""
+
+from __future__ import annotations
+
+import os
+import shlex
+import subprocess
+import sys
+import sysconfig
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+ROOT = PKG.parent
+CSRC = PKG / "csrc"
+INCLUDE = ROOT / "include"
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+CXX = os.environ.get("CXX", "g++")
+ARCH = "gfx950"
+
+# Packed-FP32 VALU forms are kept out of the generated code.
+HIP_FLAGS = [
+    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-munsafe-fp-atomics", "-Wall", "-Wno-unused-function",
+]
+
+LIB_HIP = PKG / "libope_hip.so"
+LIB_CORE = PKG / "libope.so"
+
+
+def _run(cmd: list[str]) -> None:
+    print("+", " ".join(shlex.quote(c) for c in cmd), flush=True)
+    subprocess.run(cmd, check=True)
+
+
+def build_hip(verbose_resources: bool = False) -> Path:
+    src = CSRC / "hip" / "ope_hip.hip"
+    cmd = [HIPCC, *HIP_FLAGS, f"-I{INCLUDE}", str(src), "-o", str(LIB_HIP)]
+    if verbose_resources:
+        cmd.insert(1, "-Rpass-analysis=kernel-resource-usage")
+    _run(cmd)
+    return LIB_HIP
+
+
+def build_all(verbose_resources: bool = False) -> None:
+    build_hip(verbose_resources)
+
+
+if __name__ == "__main__":
+    build_all(verbose_resources="-v" in sys.argv)

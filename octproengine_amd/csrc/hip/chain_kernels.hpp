@@ -1,0 +1,628 @@
+// Fused OCT processing chain for CDNA4 (gfx950).
+//
+// One kernel runs the whole per-A-scan chain of the reference's
+// CpuBackend::Impl::processData (cpu_backend.cpp:184-281) in a single pass
+// over HBM: raw samples are read once (128-bit loads), staged in LDS,
+// DC-removed, resampled by a banded tap table, windowed, multiplied by the
+// dispersion phasor, transformed by a radix-8 Stockham inverse FFT that lives
+// in registers and LDS, FPN-corrected and mapped to log/linear grayscale; only
+// the N/2 output floats are written.
+//
+// Work layout: an A-scan of N samples is owned by TPA = N/8 lanes; lane t
+// holds the 8 complex values at positions t + e*TPA (e = 0..7) throughout the
+// FFT.  Every Stockham pass reads those positions from LDS, so all LDS reads
+// are unit-stride across lanes; pass outputs are written to a padded layout
+// (one float2 of padding per 8) that keeps the stride-8 writes of the first
+// pass conflict-free.  Because a lane always handles the same positions, the
+// per-lane tables (taps, phasor, twiddles, FPN profile, background) are read
+// once per kernel launch and stay in VGPRs while the workgroup loops over
+// A-scans (persistent grid-stride loop).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ope_hip {
+
+// ------------------------------------------------------------------ complex
+
+struct cf {
+    float x, y;
+};
+
+__device__ __forceinline__ cf cadd(cf a, cf b) { return {a.x + b.x, a.y + b.y}; }
+__device__ __forceinline__ cf csub(cf a, cf b) { return {a.x - b.x, a.y - b.y}; }
+__device__ __forceinline__ cf cmul(cf a, cf b) {
+    return {fmaf(a.x, b.x, -a.y * b.y), fmaf(a.x, b.y, a.y * b.x)};
+}
+// +i*a and -i*a
+__device__ __forceinline__ cf cmul_pi(cf a) { return {-a.y, a.x}; }
+__device__ __forceinline__ cf cmul_ni(cf a) { return {a.y, -a.x}; }
+
+// Inverse (sign +) DFTs of size 2, 4, 8 in registers.
+__device__ __forceinline__ void idft2(cf& a0, cf& a1) {
+    cf s = cadd(a0, a1), d = csub(a0, a1);
+    a0 = s;
+    a1 = d;
+}
+
+__device__ __forceinline__ void idft4(cf& a0, cf& a1, cf& a2, cf& a3) {
+    cf s02 = cadd(a0, a2), d02 = csub(a0, a2);
+    cf s13 = cadd(a1, a3), d13 = cmul_pi(csub(a1, a3));
+    a0 = cadd(s02, s13);
+    a1 = cadd(d02, d13);
+    a2 = csub(s02, s13);
+    a3 = csub(d02, d13);
+}
+
+__device__ __forceinline__ void idft8(cf* v) {
+    // even / odd halves, then combine with w^m, w = exp(+i*pi/4)
+    cf e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6];
+    cf o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
+    idft4(e0, e1, e2, e3);
+    idft4(o0, o1, o2, o3);
+    const float c = 0.70710678118654752440f;
+    cf t1 = {c * (o1.x - o1.y), c * (o1.x + o1.y)};     // w   * o1
+    cf t2 = cmul_pi(o2);                                // w^2 * o2
+    cf t3 = {-c * (o3.x + o3.y), c * (o3.x - o3.y)};    // w^3 * o3
+    v[0] = cadd(e0, o0);
+    v[4] = csub(e0, o0);
+    v[1] = cadd(e1, t1);
+    v[5] = csub(e1, t1);
+    v[2] = cadd(e2, t2);
+    v[6] = csub(e2, t2);
+    v[3] = cadd(e3, t3);
+    v[7] = csub(e3, t3);
+}
+
+// ------------------------------------------------------------------ geometry
+
+__host__ __device__ constexpr int ilog2(int n) { return n <= 1 ? 0 : 1 + ilog2(n / 2); }
+
+template <int N>
+struct Geo {
+    static constexpr int LOG2N = ilog2(N);
+    static constexpr int TPA = N / 8;                       // lanes per A-scan
+    static constexpr int WG = TPA >= 256 ? TPA : 256;       // workgroup size
+    static constexpr int APW = WG / TPA;                    // A-scans per workgroup
+    static constexpr int NPASS = (LOG2N + 2) / 3;           // Stockham passes
+    static constexpr int LAST_R = N >> (3 * (NPASS - 1));   // radix of the last pass
+    static constexpr int PADN = N + N / 8;                  // padded float2 slots
+    static constexpr int HALF = N / 2;
+    static_assert((1 << LOG2N) == N, "N must be a power of two");
+    static_assert(N >= 256 && N <= 4096, "N outside 256..4096");
+};
+
+__device__ __forceinline__ int pad_idx(int i) { return i + (i >> 3); }
+
+// ------------------------------------------------------------------ arguments
+
+enum : int {
+    F_DISP = 1,      // multiply by the dispersion phasor
+    F_FPN = 4,       // subtract the FPN profile before the magnitude
+    F_BG = 8,        // subtract the post-process background
+    F_LOG = 16,      // log scaling (else linear)
+    F_AUTOMM = 32,   // log scaling with per-A-scan min/max
+};
+
+struct ChainArgs {
+    const void* in;          // raw samples [n_ascans][N]
+    float* out;              // [n_ascans][N/2]
+    long long n_ascans;
+    const int* tap_base;     // [N]   first input index of the tap window
+    const float* tap_w;      // [N][T] weights, window folded in
+    const float2* phasor;    // [N]   (cos, sin)
+    const float2* twiddle;   // [N]   exp(+2 pi i k / N)
+    const float2* fpn;       // [N/2]
+    const float* bg;         // [N/2]
+    int in_type;             // OPE_HIP_UINT8 / UINT16 / UINT32 / FLOAT32 classes
+    int flags;
+    int dc_window;
+    float ep_a, ep_b;        // fixed epilogue: out = ep_a * f(|X|^2) + ep_b
+    float coeff, addend;     // auto min/max epilogue
+    float log_c, log_h;      // v = log_c * (log2(m2) - log_h) = 10 log10(m2/(N/2))
+    float bg_w, bg_off;
+    // FPN statistics mode
+    double* acc;             // [8][N/2][3] (sum re, sum im, sum |X|^2)
+    long long row0;          // accumulated row index of A-scan 0 of this call
+    int seg_w;               // rows per segment
+    int stat_rows;           // A-scans of this call that enter the statistics
+};
+
+// input classes (numeric values match ope::DataType)
+enum : int { IN_U8 = 1, IN_U16 = 2, IN_U32 = 3, IN_F32 = 9 };
+
+// ------------------------------------------------------------------ loads
+
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+// Loads the 8 consecutive raw samples [8t, 8t+8) of A-scan a (streaming,
+// non-temporal: every input byte is read exactly once).
+__device__ __forceinline__ void load_raw(const void* in, int in_type, int n, long long a, int t,
+                                         uint32_t (&raw)[8]) {
+    if (in_type == IN_U16) {
+        const u32x4* p = reinterpret_cast<const u32x4*>(static_cast<const uint16_t*>(in) + a * n) + t;
+        u32x4 q = __builtin_nontemporal_load(p);
+        raw[0] = q.x; raw[1] = q.y; raw[2] = q.z; raw[3] = q.w;
+    } else if (in_type == IN_U8) {
+        const u32x2* p = reinterpret_cast<const u32x2*>(static_cast<const uint8_t*>(in) + a * n) + t;
+        u32x2 q = __builtin_nontemporal_load(p);
+        raw[0] = q.x; raw[1] = q.y;
+    } else {
+        const u32x4* p = reinterpret_cast<const u32x4*>(static_cast<const uint32_t*>(in) + a * n) + 2 * t;
+        u32x4 q0 = __builtin_nontemporal_load(p);
+        u32x4 q1 = __builtin_nontemporal_load(p + 1);
+        raw[0] = q0.x; raw[1] = q0.y; raw[2] = q0.z; raw[3] = q0.w;
+        raw[4] = q1.x; raw[5] = q1.y; raw[6] = q1.z; raw[7] = q1.w;
+    }
+}
+
+__device__ __forceinline__ void convert_raw(int in_type, const uint32_t (&raw)[8], float (&x)[8]) {
+    if (in_type == IN_U16) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            x[2 * i] = (float)(raw[i] & 0xffffu);
+            x[2 * i + 1] = (float)(raw[i] >> 16);
+        }
+    } else if (in_type == IN_U8) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) x[i] = (float)((raw[i >> 2] >> (8 * (i & 3))) & 0xffu);
+    } else if (in_type == IN_U32) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) x[i] = (float)raw[i];
+    } else {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) x[i] = __uint_as_float(raw[i]);
+    }
+}
+
+// ------------------------------------------------------------------ scans
+
+// Inclusive scan of v over the TPA lanes of one A-scan.  `part` is LDS
+// scratch of WG/64 entries (one per wavefront).  Called by all lanes.
+template <typename T, int TPA, int WG>
+__device__ __forceinline__ T ascan_inclusive_scan(T v, T* part) {
+    const int lane = threadIdx.x & 63;
+    constexpr int W = TPA < 64 ? TPA : 64;
+#pragma unroll
+    for (int d = 1; d < W; d <<= 1) {
+        T o = __shfl_up(v, d, 64);
+        if ((lane % W) >= d) v += o;
+    }
+    if constexpr (TPA > 64) {
+        const int wave = threadIdx.x >> 6;
+        if (lane == 63) part[wave] = v;
+        __syncthreads();
+        const int first = (threadIdx.x / TPA) * (TPA / 64);
+        T off = T(0);
+        for (int w = first; w < wave; ++w) off += part[w];
+        __syncthreads();
+        v += off;
+    }
+    return v;
+}
+
+// min/max reduction over the TPA lanes of one A-scan (result in all lanes).
+template <int TPA, int WG>
+__device__ __forceinline__ void ascan_minmax(float& mn, float& mx, float* part_mn, float* part_mx) {
+    constexpr int W = TPA < 64 ? TPA : 64;
+#pragma unroll
+    for (int d = 1; d < W; d <<= 1) {
+        mn = fminf(mn, __shfl_xor(mn, d, 64));
+        mx = fmaxf(mx, __shfl_xor(mx, d, 64));
+    }
+    if constexpr (TPA > 64) {
+        const int wave = threadIdx.x >> 6;
+        if ((threadIdx.x & 63) == 0) {
+            part_mn[wave] = mn;
+            part_mx[wave] = mx;
+        }
+        __syncthreads();
+        const int first = (threadIdx.x / TPA) * (TPA / 64);
+        mn = part_mn[first];
+        mx = part_mx[first];
+        for (int w = 1; w < TPA / 64; ++w) {
+            mn = fminf(mn, part_mn[first + w]);
+            mx = fmaxf(mx, part_mx[first + w]);
+        }
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------------ kernel
+
+// Compile-time modes of the chain kernel.
+enum : int {
+    M_DC = 1,      // rolling-average DC removal prologue
+    M_STATS = 2,   // FPN determination: accumulate segment statistics, no output
+};
+
+// Rolling-average DC removal of the lane's 8 samples (cpu_kernels.tpp:56-82).
+// Window sums come from exact prefix sums (int for 8/16-bit input, double
+// otherwise) built in LDS scratch `P` (N+1 entries).
+template <int N, int TPA, int WG, typename S>
+__device__ __forceinline__ void dc_remove(float (&xv)[8], int t, int window, S* P, S* part) {
+    S loc[8];
+    S s = S(0);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        s += (S)xv[i];
+        loc[i] = s;
+    }
+    const S incl = ascan_inclusive_scan<S, TPA, WG>(s, part);
+    const S excl = incl - s;
+    if (t == 0) P[0] = S(0);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) P[8 * t + i + 1] = excl + loc[i];
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const int n = 8 * t + i;
+        int start, end;
+        if (window >= 1) {
+            start = (n >= window - 1) ? n - (window - 1) : 0;
+            end = (n + window < N - 1) ? n + window : N - 1;
+        } else {
+            start = 0;
+            end = n;
+        }
+        const S sum = P[end + 1] - P[start];
+        const int cnt = end - start + 1;
+        // x - sum/cnt with a single rounding of the result
+        if constexpr (sizeof(S) == 4)
+            xv[i] = (float)((int)xv[i] * cnt - sum) / (float)cnt;  // numerator exact
+        else
+            xv[i] = (float)((double)xv[i] - (double)sum / (double)cnt);
+    }
+}
+
+// T = taps per output sample: 1 (no resampling), 2 (linear), 4 (cubic),
+// 16 (Lanczos-8).  MODE: M_* bits.
+template <int N, int T, int MODE>
+__global__ void __launch_bounds__(Geo<N>::WG) chain_kernel(ChainArgs A) {
+    using G = Geo<N>;
+    constexpr int TPA = G::TPA;
+    constexpr int APW = G::APW;
+    constexpr int NP = G::NPASS;
+    constexpr bool DC = (MODE & M_DC) != 0;
+    constexpr bool STATS = (MODE & M_STATS) != 0;
+    constexpr bool X_ALIAS_B = (NP % 2) == 0;  // x staging can share buffer B
+    constexpr int RSTRIDE = TPA + TPA / 8;     // padded distance of e -> e+1
+
+    __shared__ float2 bufA[APW * G::PADN];
+    __shared__ float2 bufB[APW * G::PADN];
+    __shared__ float xsep[X_ALIAS_B ? 1 : APW * N];
+    __shared__ double scan_part[DC ? G::WG / 64 : 1];
+    __shared__ float mm_part[STATS ? 1 : 2 * (G::WG / 64)];
+
+    const int slot = threadIdx.x / TPA;
+    const int t = threadIdx.x - slot * TPA;
+    float2* const pa = bufA + slot * G::PADN;
+    float2* const pb = bufB + slot * G::PADN;
+    float* const xs = X_ALIAS_B ? reinterpret_cast<float*>(pb) : (xsep + slot * N);
+
+    const int flags = A.flags;
+    const int in_type = A.in_type;
+
+    // ---- per-lane tables, read once per launch
+    int tb[8];
+    float tw8[8][T];
+    cf ph[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        const int j = t + e * TPA;
+        tb[e] = A.tap_base[j];
+#pragma unroll
+        for (int i = 0; i < T; ++i) tw8[e][i] = A.tap_w[j * T + i];
+        if (flags & F_DISP) {
+            float2 p = A.phasor[j];
+            ph[e] = {p.x, p.y};
+        } else {
+            ph[e] = {1.f, 0.f};
+        }
+    }
+    // twiddles of passes 1..NP-1 (pass 0 has Ns = 1: all ones); elements
+    // with r == 0 need none.
+    cf tw[NP][8];
+#pragma unroll
+    for (int p = 1; p < NP; ++p) {
+        const int R = (p < NP - 1) ? 8 : G::LAST_R;
+        const int RQ = 8 / R;
+        const int ns = 1 << (3 * p);
+        const int step = N / (ns * R);
+#pragma unroll
+        for (int e = RQ; e < 8; ++e) {
+            const int q = e % RQ, r = e / RQ;
+            const int b = t + q * TPA;
+            const int ex = (r * (b & (ns - 1)) * step) & (N - 1);
+            float2 w = A.twiddle[ex];
+            tw[p][e] = {w.x, w.y};
+        }
+    }
+    cf fp[4];
+    float bgv[4];
+    if constexpr (!STATS) {
+#pragma unroll
+        for (int o = 0; o < 4; ++o) {
+            const int k = t + o * TPA;
+            if (flags & F_FPN) {
+                float2 f = A.fpn[k];
+                fp[o] = {f.x, f.y};
+            } else {
+                fp[o] = {0.f, 0.f};
+            }
+            bgv[o] = (flags & F_BG) ? __fadd_rn(__fmul_rn(A.bg_w, A.bg[k]), A.bg_off) : 0.f;
+        }
+    }
+
+    // LDS bases: every pass reads positions t + e*TPA (base + e*RSTRIDE)
+    const int rbase = pad_idx(t);
+
+    long long a_begin, a_step, a_end;
+    if constexpr (STATS) {
+        // contiguous chunks per workgroup so segment runs are long
+        const long long n_work = A.stat_rows;
+        const long long groups = (n_work + APW - 1) / APW;
+        const long long per = (groups + gridDim.x - 1) / gridDim.x;
+        a_begin = (long long)blockIdx.x * per * APW;
+        a_end = a_begin + per * APW;
+        if (a_end > n_work) a_end = n_work;
+        a_step = APW;
+    } else {
+        a_begin = (long long)blockIdx.x * APW;
+        a_end = A.n_ascans;
+        a_step = (long long)gridDim.x * APW;
+    }
+    double sx[4] = {0, 0, 0, 0}, sy[4] = {0, 0, 0, 0}, sxx[4] = {0, 0, 0, 0};
+    int cur_seg = -1;
+
+    uint32_t raw[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (a_begin + slot < a_end) load_raw(A.in, in_type, N, a_begin + slot, t, raw);
+
+    for (long long a0 = a_begin; a0 < a_end; a0 += a_step) {
+        const long long a = a0 + slot;
+        const bool active = a < a_end;
+
+        // ---- stage converted samples in LDS
+        float xv[8];
+        convert_raw(in_type, raw, xv);
+        if constexpr (!DC) {
+            float4* dst = reinterpret_cast<float4*>(xs + 8 * t);
+            dst[0] = make_float4(xv[0], xv[1], xv[2], xv[3]);
+            dst[1] = make_float4(xv[4], xv[5], xv[6], xv[7]);
+        }
+        // prefetch the next A-scan while this one is processed
+        {
+            const long long an = a + a_step;
+            if (an < a_end) load_raw(A.in, in_type, N, an, t, raw);
+        }
+
+        if constexpr (DC) {
+            // Prefix sums go to buffer A; the previous A-scan's last pass may
+            // still be reading A, so wait for it first.
+            __syncthreads();
+            if (in_type == IN_U8 || in_type == IN_U16)
+                dc_remove<N, TPA, G::WG, int>(xv, t, A.dc_window, reinterpret_cast<int*>(pa),
+                                              reinterpret_cast<int*>(scan_part));
+            else
+                dc_remove<N, TPA, G::WG, double>(xv, t, A.dc_window, reinterpret_cast<double*>(pa),
+                                                 scan_part);
+            float4* dst = reinterpret_cast<float4*>(xs + 8 * t);
+            dst[0] = make_float4(xv[0], xv[1], xv[2], xv[3]);
+            dst[1] = make_float4(xv[4], xv[5], xv[6], xv[7]);
+        }
+        __syncthreads();
+
+        // ---- resample (banded taps, window folded in) and dispersion
+        cf v[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const float* src = xs + tb[e];
+            float acc = 0.f;
+#pragma unroll
+            for (int i = 0; i < T; ++i) acc = fmaf(tw8[e][i], src[i], acc);
+            v[e] = {acc * ph[e].x, acc * ph[e].y};
+        }
+
+        // ---- inverse FFT: NP Stockham passes
+#pragma unroll
+        for (int p = 0; p < NP; ++p) {
+            const int R = (p < NP - 1) ? 8 : G::LAST_R;
+            const int RQ = 8 / R;
+            const int ns = 1 << (3 * p);
+            float2* src = (p % 2 == 1) ? pa : pb;   // pass p>0 reads what p-1 wrote
+            float2* dst = (p % 2 == 0) ? pa : pb;
+            if (p > 0) {
+                const float2* rp = src + rbase;
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    float2 w = rp[e * RSTRIDE];
+                    v[e] = {w.x, w.y};
+                    if (e >= RQ) v[e] = cmul(v[e], tw[p][e]);
+                }
+            }
+            if (R == 8) {
+                idft8(v);
+            } else if (R == 4) {
+                idft4(v[0], v[2], v[4], v[6]);
+                idft4(v[1], v[3], v[5], v[7]);
+            } else {
+                idft2(v[0], v[4]);
+                idft2(v[1], v[5]);
+                idft2(v[2], v[6]);
+                idft2(v[3], v[7]);
+            }
+            if (p < NP - 1) {
+                // Stockham placement of output m of butterfly b:
+                //   (b / ns) * ns * R + b % ns + m * ns, padded
+#pragma unroll
+                for (int q = 0; q < RQ; ++q) {
+                    const int b = t + q * TPA;
+                    float2* wp = dst + pad_idx((b / ns) * ns * R + (b & (ns - 1)));
+#pragma unroll
+                    for (int m = 0; m < R; ++m) {
+                        const int e = q + RQ * m;
+                        const int off = (ns == 1) ? m : m * (ns + ns / 8);
+                        wp[off] = make_float2(v[e].x, v[e].y);
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        // after the last pass v[e] holds X[t + e*TPA]; e < 4 are the kept bins
+
+        if constexpr (STATS) {
+            if (active) {
+                const long long row = A.row0 + a;
+                const int seg = (int)(row / A.seg_w);
+                if (seg != cur_seg) {
+                    if (cur_seg >= 0) {
+#pragma unroll
+                        for (int o = 0; o < 4; ++o) {
+                            double* d = A.acc + ((size_t)cur_seg * G::HALF + t + o * TPA) * 3;
+                            unsafeAtomicAdd(d, sx[o]);
+                            unsafeAtomicAdd(d + 1, sy[o]);
+                            unsafeAtomicAdd(d + 2, sxx[o]);
+                            sx[o] = sy[o] = sxx[o] = 0.0;
+                        }
+                    }
+                    cur_seg = seg;
+                }
+#pragma unroll
+                for (int o = 0; o < 4; ++o) {
+                    const double re = v[o].x, im = v[o].y;
+                    sx[o] += re;
+                    sy[o] += im;
+                    sxx[o] += re * re + im * im;
+                }
+            }
+        } else {
+            // ---- epilogue: FPN, magnitude, grayscale map, background
+            float m2[4];
+#pragma unroll
+            for (int o = 0; o < 4; ++o) {
+                const cf z = csub(v[o], fp[o]);
+                m2[o] = fmaf(z.x, z.x, z.y * z.y);
+            }
+            float res[4];
+            if (flags & F_LOG) {
+                if (flags & F_AUTOMM) {
+                    float val[4];
+                    float mn = INFINITY, mx = -INFINITY;
+#pragma unroll
+                    for (int o = 0; o < 4; ++o) {
+                        val[o] = A.log_c * (__log2f(m2[o]) - A.log_h);
+                        mn = fminf(mn, val[o]);
+                        mx = fmaxf(mx, val[o]);
+                    }
+                    ascan_minmax<TPA, G::WG>(mn, mx, mm_part, mm_part + G::WG / 64);
+                    float range = mx - mn;
+                    if (range < 1e-6f) range = 1.0f;
+#pragma unroll
+                    for (int o = 0; o < 4; ++o) res[o] = A.coeff * ((val[o] - mn) / range + A.addend);
+                } else {
+#pragma unroll
+                    for (int o = 0; o < 4; ++o) res[o] = fmaf(A.ep_a, __log2f(m2[o]), A.ep_b);
+                }
+            } else {
+#pragma unroll
+                for (int o = 0; o < 4; ++o) res[o] = fmaf(A.ep_a, __fsqrt_rn(m2[o]), A.ep_b);
+            }
+            if (active) {
+                float* dst = A.out + a * G::HALF + t;
+#pragma unroll
+                for (int o = 0; o < 4; ++o) {
+                    const float r = (flags & F_BG) ? res[o] - bgv[o] : res[o];
+                    __builtin_nontemporal_store(r, dst + o * TPA);
+                }
+            }
+        }
+    }
+
+    if constexpr (STATS) {
+        if (cur_seg >= 0) {
+#pragma unroll
+            for (int o = 0; o < 4; ++o) {
+                double* d = A.acc + ((size_t)cur_seg * G::HALF + t + o * TPA) * 3;
+                unsafeAtomicAdd(d, sx[o]);
+                unsafeAtomicAdd(d + 1, sy[o]);
+                unsafeAtomicAdd(d + 2, sxx[o]);
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------ FPN select
+
+// Minimum-variance segment mean per bin, the float/double mixture of
+// getMinimumVarianceMean (cpu_kernels.tpp:402-458).  acc is [segs][width][3].
+__global__ void fpn_select_kernel(const double* __restrict__ acc, int width, int segments,
+                                  int seg_w, float2* __restrict__ out) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= width) return;
+    const float factor = 1.0f / (float)seg_w;
+    float min_var = INFINITY, bx = 0.f, by = 0.f;
+    for (int s = 0; s < segments; ++s) {
+        const double* d = acc + ((size_t)s * width + k) * 3;
+        const float mx = (float)(d[0] * (double)factor);
+        const float my = (float)(d[1] * (double)factor);
+        const float var = (float)(d[2] * (double)factor - ((double)mx * (double)mx + (double)my * (double)my));
+        if (var < min_var) {
+            min_var = var;
+            bx = mx;
+            by = my;
+        }
+    }
+    out[k] = make_float2(bx, by);
+}
+
+// Segment sums over arbitrary interleaved complex rows [height][width]
+// (for the individual getMinimumVarianceMean operation).
+__global__ void segment_sums_kernel(const float2* __restrict__ in, int width, int seg_w,
+                                    int segments, double* __restrict__ acc) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int s = blockIdx.y;
+    if (k >= width || s >= segments) return;
+    double sx = 0.0, sy = 0.0, sxx = 0.0;
+    for (int r = 0; r < seg_w; ++r) {
+        const float2 c = in[(size_t)(s * seg_w + r) * width + k];
+        const double dx = c.x, dy = c.y;
+        sx += dx;
+        sy += dy;
+        sxx += dx * dx + dy * dy;
+    }
+    double* d = acc + ((size_t)s * width + k) * 3;
+    d[0] = sx;
+    d[1] = sy;
+    d[2] = sxx;
+}
+
+// ------------------------------------------------------------------ small kernels
+
+__global__ void fill_zero_f64(double* __restrict__ p, long long n) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0;
+}
+
+// Column mean of the processed frame (cpu_backend.cpp:105-118), accumulated
+// in double.
+__global__ void bg_record_kernel(const float* __restrict__ out, long long rows, int width,
+                                 float* __restrict__ bg) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= width) return;
+    double s = 0.0;
+    for (long long r = 0; r < rows; ++r) s += out[r * width + k];
+    bg[k] = (float)(s / (double)rows);
+}
+
+// out -= weight*bg + offset (cpu_kernels.tpp:380-400)
+__global__ void bg_subtract_kernel(float* __restrict__ out, long long total, int width,
+                                   const float* __restrict__ bg, float w, float off) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const float b = __fadd_rn(__fmul_rn(w, bg[i % width]), off);
+    out[i] = out[i] - b;
+}
+
+}  // namespace ope_hip

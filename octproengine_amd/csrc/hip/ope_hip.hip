@@ -1,0 +1,843 @@
+// Host side of the C ABI declared in include/ope_hip.h.
+//
+// Owns one GPU context: device tables (banded resampling taps with the window
+// folded in, dispersion phasor, FFT twiddles, FPN profile, background
+// profile), the FPN segment-statistics accumulator, and `n_slots` frame
+// slots for host frames.  Host frames run H2D on a copy-in stream, all
+// kernels in submission order on one compute stream, and D2H on a copy-out
+// stream, chained with events so copies of one frame overlap compute of
+// another while the chain state (FPN accumulation) stays in frame order.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../../include/ope_hip.h"
+#include "chain_kernels.hpp"
+
+using namespace ope_hip;
+
+namespace {
+
+constexpr int kFpnSegments = 8;  // cpu_backend.cpp:306
+
+struct Slot {
+    void* d_in = nullptr;
+    float* d_out = nullptr;
+    hipEvent_t h2d_done = nullptr;
+    hipEvent_t comp_done = nullptr;
+    hipEvent_t d2h_done = nullptr;
+    bool used = false;
+};
+
+struct HostCall {
+    void (*fn)(void*, float*);
+    void* user;
+    float* out;
+    void (*free_fn)(void*);
+    void* free_user;
+};
+
+}  // namespace
+
+struct ope_hip_ctx {
+    int device = 0;
+    ope_hip_params p{};
+    int n = 0;
+    int taps = 1;
+    int cu_count = 0;
+    std::string err;
+
+    // host copies of the curves as last given
+    std::vector<float> curve_resample, curve_window, curve_phasor;
+    bool phasor_valid = false;
+
+    // device tables
+    int* d_tap_base = nullptr;
+    float* d_tap_w = nullptr;
+    size_t tap_w_cap = 0;
+    float2* d_phasor = nullptr;
+    float2* d_twiddle = nullptr;
+    float2* d_fpn = nullptr;
+    float* d_bg = nullptr;
+    double* d_acc = nullptr;
+    bool have_fpn = false;
+    bool have_bg = false;
+    size_t fpn_pairs_stored = 0;
+
+    // FPN accumulation state (cpu_backend.cpp:57-59,305-357)
+    long long fpn_count = 0;
+    bool fpn_requested = false;
+    bool bg_requested = false;
+
+    // streams and frame slots
+    hipStream_t s_in = nullptr, s_comp = nullptr, s_out = nullptr;
+    std::vector<Slot> slots;
+    int next_slot = 0;
+    std::vector<HostCall> calls;  // per slot
+    std::mutex submit_mu;
+};
+
+namespace {
+
+int fail(ope_hip_ctx* c, int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    if (c) c->err = buf;
+    return code;
+}
+
+thread_local std::string g_create_err;
+
+#define HIP_TRY(ctx, expr)                                                                     \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            return fail((ctx), OPE_HIP_ERUNTIME, "%s failed at %s:%d: %s", #expr, __FILE__,    \
+                        __LINE__, hipGetErrorString(e_));                                      \
+    } while (0)
+
+bool supported_n(int n) { return n == 256 || n == 512 || n == 1024 || n == 2048 || n == 4096; }
+
+int input_class(int t) {
+    switch (t) {
+        case OPE_HIP_UINT8:
+        case OPE_HIP_INT8:
+            return IN_U8;
+        case OPE_HIP_UINT16:
+        case OPE_HIP_INT16:
+            return IN_U16;
+        case OPE_HIP_UINT32:
+        case OPE_HIP_INT32:
+            return IN_U32;
+        case OPE_HIP_FLOAT32:
+            return IN_F32;
+        default:
+            return 0;
+    }
+}
+
+size_t input_bytes(int t) {
+    switch (input_class(t)) {
+        case IN_U8: return 1;
+        case IN_U16: return 2;
+        default: return 4;
+    }
+}
+
+int taps_for(const ope_hip_params& p) {
+    if (!p.resampling_enabled) return 1;
+    switch (p.interpolation) {
+        case OPE_HIP_INTERP_LINEAR: return 2;
+        case OPE_HIP_INTERP_CUBIC: return 4;
+        default: return 16;
+    }
+}
+
+// C static_cast<int>(float): truncation toward zero.
+inline int trunc_int(float v) { return static_cast<int>(v); }
+
+double lanczos8(double d) {
+    // cpu_kernels.tpp:148-162 evaluated in double
+    if (std::fabs(d) < 1e-5) return 1.0;
+    if (std::fabs(d) >= 8.0) return 0.0;
+    const double pd = M_PI * d;
+    const double pda = pd / 8.0;
+    return (std::sin(pd) / pd) * (std::sin(pda) / pda);
+}
+
+// Banded tap table: y[j] = sum_i w[j][i] * x[base[j] + i], with the window
+// folded in.  Index clamping follows cpu_kernels.tpp:84-196 (upper clamp to
+// N-1 everywhere, lower clamp to 0 for Lanczos); indices below 0, which the
+// reference reads out of bounds, are clamped to 0 here.
+int build_taps(ope_hip_ctx* c, std::vector<int>& base, std::vector<float>& w) {
+    const int n = c->n, T = c->taps;
+    const bool win = c->p.windowing_enabled != 0;
+    base.assign(n, 0);
+    w.assign((size_t)n * T, 0.f);
+    const float* curve = c->curve_resample.data();
+    const float* window = c->curve_window.data();
+    std::vector<double> acc(T);
+    std::vector<int> idx(16);
+    std::vector<double> cw(16);
+    for (int j = 0; j < n; ++j) {
+        int cnt = 0;
+        if (T == 1) {
+            idx[0] = j;
+            cw[0] = 1.0;
+            cnt = 1;
+        } else {
+            const float r = curve[j];
+            if (!std::isfinite(r)) return fail(c, OPE_HIP_EINVAL, "non-finite resampling position at %d", j);
+            if (T == 2) {
+                int n1 = trunc_int(r), n2 = n1 + 1;
+                n1 = std::min(n1, n - 1);
+                n2 = std::min(n2, n - 1);
+                const double t = (double)r - (double)n1;
+                idx[0] = n1; cw[0] = 1.0 - t;
+                idx[1] = n2; cw[1] = t;
+                cnt = 2;
+            } else if (T == 4) {
+                int n1 = trunc_int(r);
+                int n0 = std::abs(n1 - 1), n2 = n1 + 1, n3 = n2 + 1;
+                n0 = std::min(n0, n - 1);
+                n1 = std::min(n1, n - 1);
+                n2 = std::min(n2, n - 1);
+                n3 = std::min(n3, n - 1);
+                const double t = (double)r - (double)n1;
+                const double t2 = t * t, t3 = t2 * t;
+                idx[0] = n0; cw[0] = -0.5 * t3 + t2 - 0.5 * t;
+                idx[1] = n1; cw[1] = 1.5 * t3 - 2.5 * t2 + 1.0;
+                idx[2] = n2; cw[2] = -1.5 * t3 + 2.0 * t2 + 0.5 * t;
+                idx[3] = n3; cw[3] = 0.5 * t3 - 0.5 * t2;
+                cnt = 4;
+            } else {
+                const int n0 = trunc_int(r);
+                for (int i = -7; i <= 8; ++i) {
+                    int k = n0 + i;
+                    if (k < 0) k = 0;
+                    if (k >= n) k = n - 1;
+                    idx[cnt] = k;
+                    cw[cnt] = lanczos8((double)r - (double)k);
+                    ++cnt;
+                }
+            }
+        }
+        int lo = n, hi = -1;
+        for (int i = 0; i < cnt; ++i) {
+            idx[i] = std::max(0, std::min(idx[i], n - 1));
+            lo = std::min(lo, idx[i]);
+            hi = std::max(hi, idx[i]);
+        }
+        int b = std::max(0, std::min(lo, n - T));
+        if (hi - b >= T) return fail(c, OPE_HIP_EINVAL, "tap window overflow at %d", j);
+        std::fill(acc.begin(), acc.end(), 0.0);
+        for (int i = 0; i < cnt; ++i) acc[idx[i] - b] += cw[i];
+        const double wj = win ? (double)window[j] : 1.0;
+        base[j] = b;
+        for (int i = 0; i < T; ++i) w[(size_t)j * T + i] = (float)(acc[i] * wj);
+    }
+    return OPE_HIP_OK;
+}
+
+int upload_taps(ope_hip_ctx* c) {
+    c->taps = taps_for(c->p);
+    std::vector<int> base;
+    std::vector<float> w;
+    int rc = build_taps(c, base, w);
+    if (rc) return rc;
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, hipStreamSynchronize(c->s_comp));
+    if (w.size() > c->tap_w_cap) {
+        if (c->d_tap_w) HIP_TRY(c, hipFree(c->d_tap_w));
+        c->d_tap_w = nullptr;
+        HIP_TRY(c, hipMalloc(&c->d_tap_w, w.size() * sizeof(float)));
+        c->tap_w_cap = w.size();
+    }
+    HIP_TRY(c, hipMemcpy(c->d_tap_base, base.data(), base.size() * sizeof(int), hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(c->d_tap_w, w.data(), w.size() * sizeof(float), hipMemcpyHostToDevice));
+    return OPE_HIP_OK;
+}
+
+int validate_params(ope_hip_ctx* c, const ope_hip_params* p) {
+    if (!p) return fail(c, OPE_HIP_EINVAL, "params is null");
+    if (!supported_n(p->signal_length))
+        return fail(c, OPE_HIP_EUNSUP, "signal_length %d not supported (powers of two 256..4096)",
+                    p->signal_length);
+    if (p->ascans_per_bscan <= 0 || p->bscans_per_buffer <= 0)
+        return fail(c, OPE_HIP_EINVAL, "ascans_per_bscan and bscans_per_buffer must be > 0");
+    if (!input_class(p->input_type))
+        return fail(c, OPE_HIP_EUNSUP, "input type %d not supported", p->input_type);
+    if (p->resampling_enabled && (p->interpolation < 0 || p->interpolation > 2))
+        return fail(c, OPE_HIP_EINVAL, "unknown interpolation method %d", p->interpolation);
+    if (p->dc_removal_enabled && p->dc_window_size < 0)
+        return fail(c, OPE_HIP_EINVAL, "rolling-average window size must be >= 0");
+    if (p->fpn_bscan_count < 1) return fail(c, OPE_HIP_EINVAL, "fpn_bscan_count must be >= 1");
+    return OPE_HIP_OK;
+}
+
+// ---------------------------------------------------------------- launches
+
+using KernelFn = void (*)(ChainArgs);
+
+template <int N, int MODE>
+KernelFn pick_t(int taps) {
+    switch (taps) {
+        case 1: return chain_kernel<N, 1, MODE>;
+        case 2: return chain_kernel<N, 2, MODE>;
+        case 4: return chain_kernel<N, 4, MODE>;
+        default: return chain_kernel<N, 16, MODE>;
+    }
+}
+
+template <int N>
+KernelFn pick_mode(int taps, int mode) {
+    switch (mode) {
+        case 0: return pick_t<N, 0>(taps);
+        case M_DC: return pick_t<N, M_DC>(taps);
+        case M_STATS: return pick_t<N, M_STATS>(taps);
+        default: return pick_t<N, M_DC | M_STATS>(taps);
+    }
+}
+
+KernelFn pick_kernel(int n, int taps, int mode, int* wg, int* apw) {
+    switch (n) {
+        case 256: *wg = Geo<256>::WG; *apw = Geo<256>::APW; return pick_mode<256>(taps, mode);
+        case 512: *wg = Geo<512>::WG; *apw = Geo<512>::APW; return pick_mode<512>(taps, mode);
+        case 1024: *wg = Geo<1024>::WG; *apw = Geo<1024>::APW; return pick_mode<1024>(taps, mode);
+        case 2048: *wg = Geo<2048>::WG; *apw = Geo<2048>::APW; return pick_mode<2048>(taps, mode);
+        default: *wg = Geo<4096>::WG; *apw = Geo<4096>::APW; return pick_mode<4096>(taps, mode);
+    }
+}
+
+ChainArgs base_args(ope_hip_ctx* c) {
+    ChainArgs a{};
+    const ope_hip_params& p = c->p;
+    const int n = c->n, half = n / 2;
+    a.tap_base = c->d_tap_base;
+    a.tap_w = c->d_tap_w;
+    a.phasor = c->d_phasor;
+    a.twiddle = c->d_twiddle;
+    a.fpn = c->d_fpn;
+    a.bg = c->d_bg;
+    a.in_type = input_class(p.input_type);
+    a.dc_window = p.dc_window_size;
+    int flags = 0;
+    if (p.dispersion_enabled && c->phasor_valid) flags |= F_DISP;
+    if (p.log_scaling) flags |= F_LOG;
+    a.flags = flags;
+    // epilogue constants (cpu_kernels.tpp:301-378), folded in double:
+    //   log: out = coeff*((10 log10(m2/(N/2)) - min)/R + addend)
+    //   lin: out = coeff*((sqrt(m2)/(N/2) - min)/R + addend)
+    const double coeff = p.multiplicator, gmin = p.grayscale_min, add = p.addend;
+    double range = (double)(float)(p.grayscale_max - p.grayscale_min);
+    if (range < 1e-6) range = 1.0;
+    const double c10 = 10.0 * std::log10(2.0);
+    const double h = std::log2((double)half);
+    if (p.log_scaling) {
+        a.ep_a = (float)(coeff * c10 / range);
+        a.ep_b = (float)(coeff * (add - (c10 * h + gmin) / range));
+        if (p.grayscale_min == p.grayscale_max) a.flags |= F_AUTOMM;
+    } else {
+        a.ep_a = (float)(coeff / (range * (double)half));
+        a.ep_b = (float)(coeff * (add - gmin / range));
+    }
+    a.coeff = p.multiplicator;
+    a.addend = p.addend;
+    a.log_c = (float)c10;
+    a.log_h = (float)h;
+    a.bg_w = p.bg_weight;
+    a.bg_off = p.bg_offset;
+    a.acc = c->d_acc;
+    return a;
+}
+
+int launch_chain(ope_hip_ctx* c, const ChainArgs& a, bool stats, long long work_ascans, hipStream_t s) {
+    int wg = 0, apw = 1;
+    const int mode = (c->p.dc_removal_enabled ? M_DC : 0) | (stats ? M_STATS : 0);
+    KernelFn k = pick_kernel(c->n, c->taps, mode, &wg, &apw);
+    int per_cu = 0;
+    HIP_TRY(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(k), wg, 0));
+    if (per_cu < 1) per_cu = 1;
+    long long groups = (work_ascans + apw - 1) / apw;
+    long long grid = std::min<long long>(groups, (long long)per_cu * c->cu_count);
+    if (grid < 1) return OPE_HIP_OK;
+    hipLaunchKernelGGL(k, dim3((unsigned)grid), dim3(wg), 0, s, a);
+    HIP_TRY(c, hipGetLastError());
+    return OPE_HIP_OK;
+}
+
+int zero_acc(ope_hip_ctx* c, hipStream_t s) {
+    const long long n = (long long)kFpnSegments * (c->n / 2) * 3;
+    hipLaunchKernelGGL(fill_zero_f64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, c->d_acc, n);
+    HIP_TRY(c, hipGetLastError());
+    return OPE_HIP_OK;
+}
+
+// One frame of the chain on stream s (cpu_backend.cpp:169-357).
+int run_frame(ope_hip_ctx* c, const void* d_in, float* d_out, long long n_ascans, hipStream_t s) {
+    if (n_ascans <= 0) return OPE_HIP_OK;
+    const ope_hip_params& p = c->p;
+    const int half = c->n / 2;
+    ChainArgs a = base_args(c);
+    a.in = d_in;
+    a.out = d_out;
+    a.n_ascans = n_ascans;
+
+    // FPN accumulation: only the first `required` accumulated A-scans ever
+    // enter a determination (cpu_backend.cpp:324-330), and of those only
+    // the 8 full segments (cpu_kernels.tpp:414-440).
+    const long long required = (long long)p.fpn_bscan_count * p.ascans_per_bscan;
+    const int seg_w = (int)(required / kFpnSegments);
+    const long long rows_used = (long long)seg_w * kFpnSegments;
+    if (seg_w > 0 && c->fpn_count < rows_used) {
+        ChainArgs st = a;
+        st.flags = a.flags & F_DISP;
+        st.row0 = c->fpn_count;
+        st.seg_w = seg_w;
+        st.stat_rows = (int)std::min<long long>(n_ascans, rows_used - c->fpn_count);
+        int rc = launch_chain(c, st, true, st.stat_rows, s);
+        if (rc) return rc;
+    }
+    c->fpn_count += n_ascans;
+    if ((c->fpn_requested || p.fpn_continuous) && c->fpn_count >= required) {
+        if (seg_w > 0) {
+            hipLaunchKernelGGL(fpn_select_kernel, dim3((half + 255) / 256), dim3(256), 0, s, c->d_acc, half,
+                               kFpnSegments, seg_w, c->d_fpn);
+            HIP_TRY(c, hipGetLastError());
+        } else {
+            // fewer than 8 rows per segment: the reference divides by zero;
+            // its profile is then NaN.
+            std::vector<float2> nanv(half, make_float2(NAN, NAN));
+            HIP_TRY(c, hipMemcpyAsync(c->d_fpn, nanv.data(), half * sizeof(float2), hipMemcpyHostToDevice, s));
+            HIP_TRY(c, hipStreamSynchronize(s));
+        }
+        c->have_fpn = true;
+        c->fpn_pairs_stored = half;
+        if (p.fpn_continuous) {
+            c->fpn_count = std::min(c->fpn_count, required);
+        } else {
+            c->fpn_count = 0;
+            int rc = zero_acc(c, s);
+            if (rc) return rc;
+        }
+        c->fpn_requested = false;
+    }
+    if (p.fpn_removal && c->have_fpn) a.flags |= F_FPN;
+
+    const bool record_bg = p.bg_removal && c->bg_requested;
+    if (p.bg_removal && c->have_bg && !record_bg) a.flags |= F_BG;
+    int rc = launch_chain(c, a, false, n_ascans, s);
+    if (rc) return rc;
+    if (record_bg) {
+        hipLaunchKernelGGL(bg_record_kernel, dim3((half + 255) / 256), dim3(256), 0, s, d_out, n_ascans, half,
+                           c->d_bg);
+        HIP_TRY(c, hipGetLastError());
+        const long long total = n_ascans * half;
+        hipLaunchKernelGGL(bg_subtract_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, d_out,
+                           total, half, c->d_bg, p.bg_weight, p.bg_offset);
+        HIP_TRY(c, hipGetLastError());
+        c->have_bg = true;
+        c->bg_requested = false;
+    }
+    return OPE_HIP_OK;
+}
+
+void free_ctx(ope_hip_ctx* c) {
+    if (!c) return;
+    // teardown: errors are ignored, every resource is released regardless
+    (void)hipSetDevice(c->device);
+    for (hipStream_t s : {c->s_comp, c->s_in, c->s_out})
+        if (s) (void)hipStreamSynchronize(s);
+    for (auto& s : c->slots) {
+        if (s.d_in) (void)hipFree(s.d_in);
+        if (s.d_out) (void)hipFree(s.d_out);
+        if (s.h2d_done) (void)hipEventDestroy(s.h2d_done);
+        if (s.comp_done) (void)hipEventDestroy(s.comp_done);
+        if (s.d2h_done) (void)hipEventDestroy(s.d2h_done);
+    }
+    for (void* p : {(void*)c->d_tap_base, (void*)c->d_tap_w, (void*)c->d_phasor, (void*)c->d_twiddle,
+                    (void*)c->d_fpn, (void*)c->d_bg, (void*)c->d_acc})
+        if (p) (void)hipFree(p);
+    for (hipStream_t s : {c->s_in, c->s_comp, c->s_out})
+        if (s) (void)hipStreamDestroy(s);
+    delete c;
+}
+
+void host_input_free_cb(void* u) {
+    HostCall* hc = static_cast<HostCall*>(u);
+    if (hc->free_fn) hc->free_fn(hc->free_user);
+}
+
+void host_done_cb(void* u) {
+    HostCall* hc = static_cast<HostCall*>(u);
+    if (hc->fn) hc->fn(hc->user, hc->out);
+}
+
+int sync_all(ope_hip_ctx* c) {
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, hipStreamSynchronize(c->s_in));
+    HIP_TRY(c, hipStreamSynchronize(c->s_comp));
+    HIP_TRY(c, hipStreamSynchronize(c->s_out));
+    return OPE_HIP_OK;
+}
+
+}  // namespace
+
+// ====================================================================== C ABI
+
+extern "C" {
+
+const char* ope_hip_version(void) { return "octproengine_amd libope_hip 0.1 (gfx950)"; }
+
+int ope_hip_device_count(int* count) {
+    if (!count) return OPE_HIP_EINVAL;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+    *count = n;
+    return OPE_HIP_OK;
+}
+
+const char* ope_hip_last_error(const ope_hip_ctx* ctx) {
+    return ctx ? ctx->err.c_str() : g_create_err.c_str();
+}
+
+int ope_hip_create(int device, const ope_hip_params* params, int n_slots, ope_hip_ctx** out_ctx) {
+    g_create_err.clear();
+    if (!out_ctx) {
+        g_create_err = "out_ctx is null";
+        return OPE_HIP_EINVAL;
+    }
+    *out_ctx = nullptr;
+    ope_hip_ctx* c = new (std::nothrow) ope_hip_ctx();
+    if (!c) {
+        g_create_err = "out of host memory";
+        return OPE_HIP_ENOMEM;
+    }
+    int rc = validate_params(c, params);
+    if (rc) {
+        g_create_err = c->err;
+        delete c;
+        return rc;
+    }
+    if (n_slots < 1 || n_slots > 8) {
+        g_create_err = "n_slots must be in 1..8";
+        delete c;
+        return OPE_HIP_EINVAL;
+    }
+    c->device = device;
+    c->p = *params;
+    c->n = params->signal_length;
+    c->taps = taps_for(c->p);
+    auto bail = [&](int code) {
+        g_create_err = c->err;
+        free_ctx(c);
+        return code;
+    };
+#define CREATE_TRY(expr)                                                                        \
+    do {                                                                                        \
+        hipError_t e_ = (expr);                                                                 \
+        if (e_ != hipSuccess) {                                                                 \
+            fail(c, e_ == hipErrorOutOfMemory ? OPE_HIP_ENOMEM : OPE_HIP_ERUNTIME, "%s: %s", #expr, \
+                 hipGetErrorString(e_));                                                        \
+            return bail(e_ == hipErrorOutOfMemory ? OPE_HIP_ENOMEM : OPE_HIP_ERUNTIME);         \
+        }                                                                                       \
+    } while (0)
+    CREATE_TRY(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    CREATE_TRY(hipGetDeviceProperties(&prop, device));
+    c->cu_count = prop.multiProcessorCount;
+    CREATE_TRY(hipStreamCreateWithFlags(&c->s_in, hipStreamNonBlocking));
+    CREATE_TRY(hipStreamCreateWithFlags(&c->s_comp, hipStreamNonBlocking));
+    CREATE_TRY(hipStreamCreateWithFlags(&c->s_out, hipStreamNonBlocking));
+    const int n = c->n, half = n / 2;
+    CREATE_TRY(hipMalloc(&c->d_tap_base, n * sizeof(int)));
+    CREATE_TRY(hipMalloc(&c->d_tap_w, (size_t)n * 16 * sizeof(float)));
+    c->tap_w_cap = (size_t)n * 16;
+    CREATE_TRY(hipMalloc(&c->d_phasor, n * sizeof(float2)));
+    CREATE_TRY(hipMalloc(&c->d_twiddle, n * sizeof(float2)));
+    CREATE_TRY(hipMalloc(&c->d_fpn, half * sizeof(float2)));
+    CREATE_TRY(hipMalloc(&c->d_bg, half * sizeof(float)));
+    CREATE_TRY(hipMalloc(&c->d_acc, (size_t)kFpnSegments * half * 3 * sizeof(double)));
+    {
+        std::vector<float2> tw(n);
+        for (int k = 0; k < n; ++k) {
+            const double ang = 2.0 * M_PI * (double)k / (double)n;
+            tw[k] = make_float2((float)std::cos(ang), (float)std::sin(ang));
+        }
+        CREATE_TRY(hipMemcpy(c->d_twiddle, tw.data(), n * sizeof(float2), hipMemcpyHostToDevice));
+        std::vector<float2> zero2(half, make_float2(0.f, 0.f));
+        CREATE_TRY(hipMemcpy(c->d_fpn, zero2.data(), half * sizeof(float2), hipMemcpyHostToDevice));
+        std::vector<float> zero1(half, 0.f);
+        CREATE_TRY(hipMemcpy(c->d_bg, zero1.data(), half * sizeof(float), hipMemcpyHostToDevice));
+        std::vector<double> zacc((size_t)kFpnSegments * half * 3, 0.0);
+        CREATE_TRY(hipMemcpy(c->d_acc, zacc.data(), zacc.size() * sizeof(double), hipMemcpyHostToDevice));
+        std::vector<float2> ph(n, make_float2(1.f, 0.f));
+        CREATE_TRY(hipMemcpy(c->d_phasor, ph.data(), n * sizeof(float2), hipMemcpyHostToDevice));
+    }
+    const size_t frame_ascans = (size_t)params->ascans_per_bscan * params->bscans_per_buffer;
+    c->slots.resize(n_slots);
+    c->calls.resize(n_slots);
+    for (auto& s : c->slots) {
+        CREATE_TRY(hipMalloc(&s.d_in, frame_ascans * n * input_bytes(params->input_type)));
+        CREATE_TRY(hipMalloc(&s.d_out, frame_ascans * half * sizeof(float)));
+        CREATE_TRY(hipEventCreateWithFlags(&s.h2d_done, hipEventDisableTiming));
+        CREATE_TRY(hipEventCreateWithFlags(&s.comp_done, hipEventDisableTiming));
+        CREATE_TRY(hipEventCreateWithFlags(&s.d2h_done, hipEventDisableTiming));
+    }
+#undef CREATE_TRY
+    // Curves start as zeros, as the CPU backend's do after initialize
+    // (cpu_backend.cpp:423-425), until the Processor uploads real ones.
+    c->curve_resample.assign(n, 0.f);
+    c->curve_window.assign(n, 0.f);
+    c->curve_phasor.assign(2 * (size_t)n, 0.f);
+    c->phasor_valid = true;
+    {
+        hipError_t e = hipMemcpy(c->d_phasor, c->curve_phasor.data(), 2 * (size_t)n * sizeof(float),
+                                 hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            fail(c, OPE_HIP_ERUNTIME, "hipMemcpy phasor: %s", hipGetErrorString(e));
+            return bail(OPE_HIP_ERUNTIME);
+        }
+    }
+    rc = upload_taps(c);
+    if (rc) return bail(rc);
+    *out_ctx = c;
+    return OPE_HIP_OK;
+}
+
+int ope_hip_destroy(ope_hip_ctx* ctx) {
+    free_ctx(ctx);
+    return OPE_HIP_OK;
+}
+
+int ope_hip_set_params(ope_hip_ctx* c, const ope_hip_params* params) {
+    if (!c) return OPE_HIP_EINVAL;
+    int rc = validate_params(c, params);
+    if (rc) return rc;
+    if (params->signal_length != c->p.signal_length || params->ascans_per_bscan != c->p.ascans_per_bscan ||
+        params->bscans_per_buffer != c->p.bscans_per_buffer || params->input_type != c->p.input_type)
+        return fail(c, OPE_HIP_EINVAL, "set_params cannot change dimensions or input type");
+    std::lock_guard<std::mutex> lk(c->submit_mu);
+    ope_hip_params old = c->p;
+    c->p = *params;
+    if (old.resampling_enabled != params->resampling_enabled || old.interpolation != params->interpolation ||
+        old.windowing_enabled != params->windowing_enabled) {
+        rc = upload_taps(c);
+        if (rc) return rc;
+    }
+    return OPE_HIP_OK;
+}
+
+int ope_hip_set_resampling_curve(ope_hip_ctx* c, const float* curve, size_t length) {
+    if (!c) return OPE_HIP_EINVAL;
+    if (!curve && length > 0) return fail(c, OPE_HIP_EINVAL, "Invalid resampling curve pointer");
+    if (length != (size_t)c->n)
+        return fail(c, OPE_HIP_EINVAL, "resampling curve has %zu values, expected %d", length, c->n);
+    std::lock_guard<std::mutex> lk(c->submit_mu);
+    c->curve_resample.assign(curve, curve + length);
+    int rc = sync_all(c);
+    if (rc) return rc;
+    return upload_taps(c);
+}
+
+int ope_hip_set_window_curve(ope_hip_ctx* c, const float* curve, size_t length) {
+    if (!c) return OPE_HIP_EINVAL;
+    if (!curve && length > 0) return fail(c, OPE_HIP_EINVAL, "Invalid window curve pointer");
+    if (length != (size_t)c->n)
+        return fail(c, OPE_HIP_EINVAL, "window curve has %zu values, expected %d", length, c->n);
+    std::lock_guard<std::mutex> lk(c->submit_mu);
+    c->curve_window.assign(curve, curve + length);
+    int rc = sync_all(c);
+    if (rc) return rc;
+    return upload_taps(c);
+}
+
+int ope_hip_set_dispersion_curve(ope_hip_ctx* c, const float* cos_sin, size_t length) {
+    if (!c) return OPE_HIP_EINVAL;
+    if (!cos_sin && length > 0) return fail(c, OPE_HIP_EINVAL, "Invalid dispersion curve pointer");
+    std::lock_guard<std::mutex> lk(c->submit_mu);
+    int rc = sync_all(c);
+    if (rc) return rc;
+    c->curve_phasor.assign(cos_sin, cos_sin + length);
+    c->phasor_valid = (length == 2 * (size_t)c->n);
+    if (c->phasor_valid)
+        HIP_TRY(c, hipMemcpy(c->d_phasor, cos_sin, length * sizeof(float), hipMemcpyHostToDevice));
+    return OPE_HIP_OK;
+}
+
+int ope_hip_set_fpn_profile(ope_hip_ctx* c, const float* interleaved, size_t pairs) {
+    if (!c) return OPE_HIP_EINVAL;
+    if (!interleaved && pairs > 0) return fail(c, OPE_HIP_EINVAL, "Invalid fixed-pattern noise profile pointer");
+    std::lock_guard<std::mutex> lk(c->submit_mu);
+    int rc = sync_all(c);
+    if (rc) return rc;
+    const int half = c->n / 2;
+    std::vector<float2> prof(half, make_float2(0.f, 0.f));
+    const size_t use = std::min<size_t>(pairs, half);
+    for (size_t k = 0; k < use; ++k) prof[k] = make_float2(interleaved[2 * k], interleaved[2 * k + 1]);
+    HIP_TRY(c, hipMemcpy(c->d_fpn, prof.data(), half * sizeof(float2), hipMemcpyHostToDevice));
+    c->have_fpn = pairs > 0;
+    c->fpn_pairs_stored = pairs;
+    return OPE_HIP_OK;
+}
+
+int ope_hip_get_fpn_profile(ope_hip_ctx* c, float* interleaved, size_t pairs_capacity, size_t* pairs_out) {
+    if (!c) return OPE_HIP_EINVAL;
+    std::lock_guard<std::mutex> lk(c->submit_mu);
+    int rc = sync_all(c);
+    if (rc) return rc;
+    const size_t half = c->n / 2;
+    const size_t stored = c->have_fpn ? std::min(c->fpn_pairs_stored, half) : 0;
+    if (pairs_out) *pairs_out = stored;
+    if (interleaved && stored) {
+        std::vector<float2> prof(half);
+        HIP_TRY(c, hipMemcpy(prof.data(), c->d_fpn, half * sizeof(float2), hipMemcpyDeviceToHost));
+        const size_t use = std::min(pairs_capacity, stored);
+        for (size_t k = 0; k < use; ++k) {
+            interleaved[2 * k] = prof[k].x;
+            interleaved[2 * k + 1] = prof[k].y;
+        }
+    }
+    return OPE_HIP_OK;
+}
+
+int ope_hip_request_fpn(ope_hip_ctx* c) {
+    if (!c) return OPE_HIP_EINVAL;
+    std::lock_guard<std::mutex> lk(c->submit_mu);
+    c->fpn_requested = true;
+    return OPE_HIP_OK;
+}
+
+int ope_hip_set_bg_profile(ope_hip_ctx* c, const float* profile, size_t length) {
+    if (!c) return OPE_HIP_EINVAL;
+    if (!profile) return fail(c, OPE_HIP_EINVAL, "Background curve data is null");
+    if (length != (size_t)(c->n / 2))
+        return fail(c, OPE_HIP_EINVAL, "Invalid background buffer size. Expected %d but got %zu", c->n / 2, length);
+    std::lock_guard<std::mutex> lk(c->submit_mu);
+    int rc = sync_all(c);
+    if (rc) return rc;
+    HIP_TRY(c, hipMemcpy(c->d_bg, profile, length * sizeof(float), hipMemcpyHostToDevice));
+    c->have_bg = true;
+    return OPE_HIP_OK;
+}
+
+int ope_hip_get_bg_profile(ope_hip_ctx* c, float* profile, size_t capacity, size_t* length_out) {
+    if (!c) return OPE_HIP_EINVAL;
+    std::lock_guard<std::mutex> lk(c->submit_mu);
+    int rc = sync_all(c);
+    if (rc) return rc;
+    const size_t half = c->n / 2;
+    const size_t stored = c->have_bg ? half : 0;
+    if (length_out) *length_out = stored;
+    if (profile && stored) {
+        std::vector<float> tmp(half);
+        HIP_TRY(c, hipMemcpy(tmp.data(), c->d_bg, half * sizeof(float), hipMemcpyDeviceToHost));
+        std::copy(tmp.begin(), tmp.begin() + std::min(capacity, stored), profile);
+    }
+    return OPE_HIP_OK;
+}
+
+int ope_hip_request_bg_recording(ope_hip_ctx* c) {
+    if (!c) return OPE_HIP_EINVAL;
+    std::lock_guard<std::mutex> lk(c->submit_mu);
+    c->bg_requested = true;
+    return OPE_HIP_OK;
+}
+
+int ope_hip_process_device(ope_hip_ctx* c, const void* d_in, float* d_out, int64_t n_ascans, void* stream) {
+    if (!c) return OPE_HIP_EINVAL;
+    if (n_ascans < 0) return fail(c, OPE_HIP_EINVAL, "n_ascans must be >= 0");
+    if (n_ascans > 0 && (!d_in || !d_out)) return fail(c, OPE_HIP_EINVAL, "null device pointer");
+    std::lock_guard<std::mutex> lk(c->submit_mu);
+    HIP_TRY(c, hipSetDevice(c->device));
+    hipStream_t s = static_cast<hipStream_t>(stream);  // NULL: the HIP null stream
+    return run_frame(c, d_in, d_out, n_ascans, s);
+}
+
+int ope_hip_submit_host(ope_hip_ctx* c, const void* host_in, float* host_out, void (*on_input_free)(void*),
+                        void* user_in, ope_hip_frame_cb on_done, void* user) {
+    if (!c) return OPE_HIP_EINVAL;
+    if (!host_in || !host_out) return fail(c, OPE_HIP_EINVAL, "null host buffer");
+    std::lock_guard<std::mutex> lk(c->submit_mu);
+    HIP_TRY(c, hipSetDevice(c->device));
+    const int si = c->next_slot;
+    c->next_slot = (c->next_slot + 1) % (int)c->slots.size();
+    Slot& s = c->slots[si];
+    if (s.used) HIP_TRY(c, hipEventSynchronize(s.d2h_done));  // slot buffers and callback record free
+    s.used = true;
+    HostCall& hc = c->calls[si];
+    hc = HostCall{on_done, user, host_out, on_input_free, user_in};
+    const long long frame = (long long)c->p.ascans_per_bscan * c->p.bscans_per_buffer;
+    const size_t in_bytes = (size_t)frame * c->n * input_bytes(c->p.input_type);
+    const size_t out_bytes = (size_t)frame * (c->n / 2) * sizeof(float);
+    HIP_TRY(c, hipMemcpyAsync(s.d_in, host_in, in_bytes, hipMemcpyHostToDevice, c->s_in));
+    // events are recorded after the host callbacks so that a slot is only
+    // reused once both of its callbacks have returned
+    HIP_TRY(c, hipLaunchHostFunc(c->s_in, host_input_free_cb, &hc));
+    HIP_TRY(c, hipEventRecord(s.h2d_done, c->s_in));
+    HIP_TRY(c, hipStreamWaitEvent(c->s_comp, s.h2d_done, 0));
+    int rc = run_frame(c, s.d_in, s.d_out, frame, c->s_comp);
+    if (rc) return rc;
+    HIP_TRY(c, hipEventRecord(s.comp_done, c->s_comp));
+    HIP_TRY(c, hipStreamWaitEvent(c->s_out, s.comp_done, 0));
+    HIP_TRY(c, hipMemcpyAsync(host_out, s.d_out, out_bytes, hipMemcpyDeviceToHost, c->s_out));
+    HIP_TRY(c, hipLaunchHostFunc(c->s_out, host_done_cb, &hc));
+    HIP_TRY(c, hipEventRecord(s.d2h_done, c->s_out));
+    return OPE_HIP_OK;
+}
+
+int ope_hip_synchronize(ope_hip_ctx* c) {
+    if (!c) return OPE_HIP_EINVAL;
+    return sync_all(c);
+}
+
+int ope_hip_op_min_variance_mean(ope_hip_ctx* c, const float* input, int width, int height, int segments,
+                                 float* out) {
+    if (!c) return OPE_HIP_EINVAL;
+    if (!input || !out) return fail(c, OPE_HIP_EINVAL, "null pointer");
+    if (width <= 0 || height <= 0 || segments <= 0)
+        return fail(c, OPE_HIP_EINVAL, "width, height and segments must be > 0");
+    const int seg_w = height / segments;
+    if (seg_w <= 0) return fail(c, OPE_HIP_EINVAL, "height must be >= segments");
+    std::lock_guard<std::mutex> lk(c->submit_mu);
+    HIP_TRY(c, hipSetDevice(c->device));
+    const size_t in_bytes = (size_t)seg_w * segments * width * 2 * sizeof(float);
+    void* d_in = nullptr;
+    double* d_acc = nullptr;
+    float2* d_out = nullptr;
+    int rc = OPE_HIP_OK;
+    auto cleanup = [&]() {
+        if (d_in) (void)hipFree(d_in);
+        if (d_acc) (void)hipFree(d_acc);
+        if (d_out) (void)hipFree(d_out);
+    };
+    hipError_t e = hipMalloc(&d_in, in_bytes);
+    if (e == hipSuccess) e = hipMalloc(&d_acc, (size_t)segments * width * 3 * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&d_out, (size_t)width * sizeof(float2));
+    if (e == hipSuccess) e = hipMemcpy(d_in, input, in_bytes, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(segment_sums_kernel, dim3((width + 255) / 256, segments), dim3(256), 0, c->s_comp,
+                           static_cast<const float2*>(d_in), width, seg_w, segments, d_acc);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(fpn_select_kernel, dim3((width + 255) / 256), dim3(256), 0, c->s_comp, d_acc, width,
+                           segments, seg_w, d_out);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(c->s_comp);
+    if (e == hipSuccess) e = hipMemcpy(out, d_out, (size_t)width * sizeof(float2), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) rc = fail(c, OPE_HIP_ERUNTIME, "min_variance_mean: %s", hipGetErrorString(e));
+    cleanup();
+    return rc;
+}
+
+int ope_hip_host_alloc(size_t bytes, void** ptr) {
+    if (!ptr) return OPE_HIP_EINVAL;
+    *ptr = nullptr;
+    if (bytes == 0) return OPE_HIP_OK;
+    if (hipHostMalloc(ptr, bytes, hipHostMallocDefault) != hipSuccess) {
+        *ptr = nullptr;
+        return OPE_HIP_ENOMEM;
+    }
+    std::memset(*ptr, 0, bytes);
+    return OPE_HIP_OK;
+}
+
+int ope_hip_host_free(void* ptr) {
+    if (ptr) (void)hipHostFree(ptr);
+    return OPE_HIP_OK;
+}
+
+}  // extern "C"
