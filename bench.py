@@ -114,6 +114,9 @@ def main():
     ap.add_argument("--ring", type=int, default=8, help="distinct frames cycled through")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a plain 256 MiB read + 256 MiB write streaming kernel, the "
+                         "reference point for the FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -171,6 +174,14 @@ def main():
     ev1.record(stream)
     torch.cuda.synchronize()
     batched = 4 * frame * reps / (ev0.elapsed_time(ev1) / 1e3)
+
+    if args.copy_calibration:
+        src = torch.ones(64 << 20, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            torch.mul(src, 1.0, out=dst)  # one elementwise kernel: 256 MiB in, 256 MiB out
+        torch.cuda.synchronize()
+        del src, dst
 
     # sanity: the last output is finite and a plausible grayscale image
     o = outs[(args.steps - 1) % args.ring]

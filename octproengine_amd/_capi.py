@@ -133,9 +133,9 @@ class Context:
             raise HipError(rc, lib.ope_hip_last_error(self._h).decode())
 
     def close(self) -> None:
-        if getattr(self, "_h", None):
+        if getattr(self, "_h", None) and lib is not None:
             lib.ope_hip_destroy(self._h)
-            self._h = None
+        self._h = None
 
     def __del__(self):
         self.close()

@@ -61,23 +61,33 @@ __device__ __forceinline__ void idft8(cf* v) {
     cf o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
     idft4(e0, e1, e2, e3);
     idft4(o0, o1, o2, o3);
+    // w * o1 = c (o1.x - o1.y, o1.x + o1.y), w^3 * o3 = c (-(o3.x + o3.y), o3.x - o3.y),
+    // c = 1/sqrt(2); the scaling by c is folded into the combining FMAs.
     const float c = 0.70710678118654752440f;
-    cf t1 = {c * (o1.x - o1.y), c * (o1.x + o1.y)};     // w   * o1
-    cf t2 = cmul_pi(o2);                                // w^2 * o2
-    cf t3 = {-c * (o3.x + o3.y), c * (o3.x - o3.y)};    // w^3 * o3
+    const cf u1 = {o1.x - o1.y, o1.x + o1.y};
+    const cf u3 = {o3.x + o3.y, o3.x - o3.y};
+    const cf t2 = cmul_pi(o2);                          // w^2 * o2
     v[0] = cadd(e0, o0);
     v[4] = csub(e0, o0);
-    v[1] = cadd(e1, t1);
-    v[5] = csub(e1, t1);
+    v[1] = {fmaf(c, u1.x, e1.x), fmaf(c, u1.y, e1.y)};
+    v[5] = {fmaf(-c, u1.x, e1.x), fmaf(-c, u1.y, e1.y)};
     v[2] = cadd(e2, t2);
     v[6] = csub(e2, t2);
-    v[3] = cadd(e3, t3);
-    v[7] = csub(e3, t3);
+    v[3] = {fmaf(-c, u3.x, e3.x), fmaf(c, u3.y, e3.y)};
+    v[7] = {fmaf(c, u3.x, e3.x), fmaf(-c, u3.y, e3.y)};
 }
 
 // ------------------------------------------------------------------ geometry
 
 __host__ __device__ constexpr int ilog2(int n) { return n <= 1 ? 0 : 1 + ilog2(n / 2); }
+
+// Padded LDS slot of complex element i (one float2 of padding per 32
+// elements, one more per 128).  Every shift is >= 3, so the Stockham read
+// (t + e*TPA) and write ((b/ns)*ns*R + b%ns + m*ns) addresses of a lane
+// split into one base register plus compile-time offsets: the terms share no
+// bits, and the padding is additive over such sums.  Among paddings of this
+// family it gave the fewest SQ_LDS_BANK_CONFLICT cycles on gfx950.
+__host__ __device__ constexpr int pad_slots(int i) { return i + 2 * (i >> 5) + (i >> 7); }
 
 template <int N>
 struct Geo {
@@ -87,13 +97,15 @@ struct Geo {
     static constexpr int APW = WG / TPA;                    // A-scans per workgroup
     static constexpr int NPASS = (LOG2N + 2) / 3;           // Stockham passes
     static constexpr int LAST_R = N >> (3 * (NPASS - 1));   // radix of the last pass
-    static constexpr int PADN = N + N / 8;                  // padded float2 slots
+    // padded float2 slots; at least N+1 so buffer A also holds the N+1
+    // prefix sums (int or double) of the DC-removal prologue
+    static constexpr int PADN = pad_slots(N - 1) + 1 > N + 1 ? pad_slots(N - 1) + 1 : N + 1;
     static constexpr int HALF = N / 2;
     static_assert((1 << LOG2N) == N, "N must be a power of two");
     static_assert(N >= 256 && N <= 4096, "N outside 256..4096");
 };
 
-__device__ __forceinline__ int pad_idx(int i) { return i + (i >> 3); }
+__device__ __forceinline__ int pad_idx(int i) { return pad_slots(i); }
 
 // ------------------------------------------------------------------ arguments
 
@@ -277,30 +289,45 @@ __device__ __forceinline__ void dc_remove(float (&xv)[8], int t, int window, S* 
     }
 }
 
+#ifndef OPE_WAVES_PER_EU
+#define OPE_WAVES_PER_EU 1
+#endif
+
 // T = taps per output sample: 1 (no resampling), 2 (linear), 4 (cubic),
-// 16 (Lanczos-8).  MODE: M_* bits.
-template <int N, int T, int MODE>
-__global__ void __launch_bounds__(Geo<N>::WG) chain_kernel(ChainArgs A) {
+// 16 (Lanczos-8).  MODE: M_* bits.  AS: A-scans each lane carries through
+// the chain at once (instruction-level parallelism: the LDS traffic of one
+// overlaps the arithmetic of the other, and each barrier covers AS A-scans;
+// the per-position tables are shared).
+template <int N, int T, int MODE, int AS>
+__global__ void __launch_bounds__(Geo<N>::WG) __attribute__((amdgpu_waves_per_eu(OPE_WAVES_PER_EU)))
+chain_kernel(ChainArgs A) {
     using G = Geo<N>;
     constexpr int TPA = G::TPA;
     constexpr int APW = G::APW;
     constexpr int NP = G::NPASS;
+    constexpr int SLOTS = APW * AS;             // A-scans per workgroup iteration
+    constexpr int NW = G::WG / 64;              // wavefronts per workgroup
     constexpr bool DC = (MODE & M_DC) != 0;
     constexpr bool STATS = (MODE & M_STATS) != 0;
     constexpr bool X_ALIAS_B = (NP % 2) == 0;  // x staging can share buffer B
-    constexpr int RSTRIDE = TPA + TPA / 8;     // padded distance of e -> e+1
 
-    __shared__ float2 bufA[APW * G::PADN];
-    __shared__ float2 bufB[APW * G::PADN];
-    __shared__ float xsep[X_ALIAS_B ? 1 : APW * N];
-    __shared__ double scan_part[DC ? G::WG / 64 : 1];
-    __shared__ float mm_part[STATS ? 1 : 2 * (G::WG / 64)];
+    __shared__ float2 bufA[SLOTS * G::PADN];
+    __shared__ float2 bufB[SLOTS * G::PADN];
+    __shared__ float xsep[X_ALIAS_B ? 1 : SLOTS * N];
+    __shared__ double scan_part[DC ? AS * NW : 1];
+    __shared__ float mm_part[STATS ? 1 : 2 * AS * NW];
 
     const int slot = threadIdx.x / TPA;
     const int t = threadIdx.x - slot * TPA;
-    float2* const pa = bufA + slot * G::PADN;
-    float2* const pb = bufB + slot * G::PADN;
-    float* const xs = X_ALIAS_B ? reinterpret_cast<float*>(pb) : (xsep + slot * N);
+    float2* pa[AS];
+    float2* pb[AS];
+    float* xs[AS];
+#pragma unroll
+    for (int u = 0; u < AS; ++u) {
+        pa[u] = bufA + (slot * AS + u) * G::PADN;
+        pb[u] = bufB + (slot * AS + u) * G::PADN;
+        xs[u] = X_ALIAS_B ? reinterpret_cast<float*>(pb[u]) : (xsep + (slot * AS + u) * N);
+    }
 
     const int flags = A.flags;
     const int in_type = A.in_type;
@@ -356,73 +383,89 @@ __global__ void __launch_bounds__(Geo<N>::WG) chain_kernel(ChainArgs A) {
         }
     }
 
-    // LDS bases: every pass reads positions t + e*TPA (base + e*RSTRIDE)
+    // LDS base: every pass reads positions t + e*TPA (base + constant)
     const int rbase = pad_idx(t);
 
     long long a_begin, a_step, a_end;
     if constexpr (STATS) {
         // contiguous chunks per workgroup so segment runs are long
         const long long n_work = A.stat_rows;
-        const long long groups = (n_work + APW - 1) / APW;
+        const long long groups = (n_work + SLOTS - 1) / SLOTS;
         const long long per = (groups + gridDim.x - 1) / gridDim.x;
-        a_begin = (long long)blockIdx.x * per * APW;
-        a_end = a_begin + per * APW;
+        a_begin = (long long)blockIdx.x * per * SLOTS;
+        a_end = a_begin + per * SLOTS;
         if (a_end > n_work) a_end = n_work;
-        a_step = APW;
+        a_step = SLOTS;
     } else {
-        a_begin = (long long)blockIdx.x * APW;
+        a_begin = (long long)blockIdx.x * SLOTS;
         a_end = A.n_ascans;
-        a_step = (long long)gridDim.x * APW;
+        a_step = (long long)gridDim.x * SLOTS;
     }
-    double sx[4] = {0, 0, 0, 0}, sy[4] = {0, 0, 0, 0}, sxx[4] = {0, 0, 0, 0};
-    int cur_seg = -1;
+    double sx[AS][4], sy[AS][4], sxx[AS][4];
+    int cur_seg[AS];
+#pragma unroll
+    for (int u = 0; u < AS; ++u) {
+        cur_seg[u] = -1;
+#pragma unroll
+        for (int o = 0; o < 4; ++o) sx[u][o] = sy[u][o] = sxx[u][o] = 0.0;
+    }
 
-    uint32_t raw[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    if (a_begin + slot < a_end) load_raw(A.in, in_type, N, a_begin + slot, t, raw);
+    uint32_t raw[AS][8];
+#pragma unroll
+    for (int u = 0; u < AS; ++u) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) raw[u][i] = 0;
+        const long long a = a_begin + slot * AS + u;
+        if (a < a_end) load_raw(A.in, in_type, N, a, t, raw[u]);
+    }
 
     for (long long a0 = a_begin; a0 < a_end; a0 += a_step) {
-        const long long a = a0 + slot;
-        const bool active = a < a_end;
-
         // ---- stage converted samples in LDS
-        float xv[8];
-        convert_raw(in_type, raw, xv);
-        if constexpr (!DC) {
-            float4* dst = reinterpret_cast<float4*>(xs + 8 * t);
-            dst[0] = make_float4(xv[0], xv[1], xv[2], xv[3]);
-            dst[1] = make_float4(xv[4], xv[5], xv[6], xv[7]);
-        }
-        // prefetch the next A-scan while this one is processed
-        {
-            const long long an = a + a_step;
-            if (an < a_end) load_raw(A.in, in_type, N, an, t, raw);
+        float xv[AS][8];
+#pragma unroll
+        for (int u = 0; u < AS; ++u) {
+            convert_raw(in_type, raw[u], xv[u]);
+            if constexpr (!DC) {
+                float4* dst = reinterpret_cast<float4*>(xs[u] + 8 * t);
+                dst[0] = make_float4(xv[u][0], xv[u][1], xv[u][2], xv[u][3]);
+                dst[1] = make_float4(xv[u][4], xv[u][5], xv[u][6], xv[u][7]);
+            }
+            // prefetch the next A-scan while this one is processed
+            const long long an = a0 + a_step + slot * AS + u;
+            if (an < a_end) load_raw(A.in, in_type, N, an, t, raw[u]);
         }
 
         if constexpr (DC) {
             // Prefix sums go to buffer A; the previous A-scan's last pass may
             // still be reading A, so wait for it first.
             __syncthreads();
-            if (in_type == IN_U8 || in_type == IN_U16)
-                dc_remove<N, TPA, G::WG, int>(xv, t, A.dc_window, reinterpret_cast<int*>(pa),
-                                              reinterpret_cast<int*>(scan_part));
-            else
-                dc_remove<N, TPA, G::WG, double>(xv, t, A.dc_window, reinterpret_cast<double*>(pa),
-                                                 scan_part);
-            float4* dst = reinterpret_cast<float4*>(xs + 8 * t);
-            dst[0] = make_float4(xv[0], xv[1], xv[2], xv[3]);
-            dst[1] = make_float4(xv[4], xv[5], xv[6], xv[7]);
+#pragma unroll
+            for (int u = 0; u < AS; ++u) {
+                if (in_type == IN_U8 || in_type == IN_U16)
+                    dc_remove<N, TPA, G::WG, int>(xv[u], t, A.dc_window, reinterpret_cast<int*>(pa[u]),
+                                                  reinterpret_cast<int*>(scan_part + u * NW));
+                else
+                    dc_remove<N, TPA, G::WG, double>(xv[u], t, A.dc_window, reinterpret_cast<double*>(pa[u]),
+                                                     scan_part + u * NW);
+                float4* dst = reinterpret_cast<float4*>(xs[u] + 8 * t);
+                dst[0] = make_float4(xv[u][0], xv[u][1], xv[u][2], xv[u][3]);
+                dst[1] = make_float4(xv[u][4], xv[u][5], xv[u][6], xv[u][7]);
+            }
         }
         __syncthreads();
 
         // ---- resample (banded taps, window folded in) and dispersion
-        cf v[8];
+        cf v[AS][8];
 #pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            const float* src = xs + tb[e];
-            float acc = 0.f;
+        for (int u = 0; u < AS; ++u) {
 #pragma unroll
-            for (int i = 0; i < T; ++i) acc = fmaf(tw8[e][i], src[i], acc);
-            v[e] = {acc * ph[e].x, acc * ph[e].y};
+            for (int e = 0; e < 8; ++e) {
+                const float* src = xs[u] + tb[e];
+                float acc = 0.f;
+#pragma unroll
+                for (int i = 0; i < T; ++i) acc = fmaf(tw8[e][i], src[i], acc);
+                v[u][e] = {acc * ph[e].x, acc * ph[e].y};
+            }
         }
 
         // ---- inverse FFT: NP Stockham passes
@@ -431,123 +474,147 @@ __global__ void __launch_bounds__(Geo<N>::WG) chain_kernel(ChainArgs A) {
             const int R = (p < NP - 1) ? 8 : G::LAST_R;
             const int RQ = 8 / R;
             const int ns = 1 << (3 * p);
-            float2* src = (p % 2 == 1) ? pa : pb;   // pass p>0 reads what p-1 wrote
-            float2* dst = (p % 2 == 0) ? pa : pb;
             if (p > 0) {
-                const float2* rp = src + rbase;
 #pragma unroll
-                for (int e = 0; e < 8; ++e) {
-                    float2 w = rp[e * RSTRIDE];
-                    v[e] = {w.x, w.y};
-                    if (e >= RQ) v[e] = cmul(v[e], tw[p][e]);
+                for (int u = 0; u < AS; ++u) {
+                    // pass p>0 reads what pass p-1 wrote
+                    const float2* rp = ((p % 2 == 1) ? pa[u] : pb[u]) + rbase;
+#pragma unroll
+                    for (int e = 0; e < 8; ++e) {
+                        // t < TPA and e*TPA share no bits, so the padded
+                        // index splits into a lane base plus a constant
+                        float2 w = rp[pad_slots(e * TPA)];
+                        v[u][e] = {w.x, w.y};
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < AS; ++u) {
+#pragma unroll
+                    for (int e = RQ; e < 8; ++e) v[u][e] = cmul(v[u][e], tw[p][e]);
                 }
             }
-            if (R == 8) {
-                idft8(v);
-            } else if (R == 4) {
-                idft4(v[0], v[2], v[4], v[6]);
-                idft4(v[1], v[3], v[5], v[7]);
-            } else {
-                idft2(v[0], v[4]);
-                idft2(v[1], v[5]);
-                idft2(v[2], v[6]);
-                idft2(v[3], v[7]);
+#pragma unroll
+            for (int u = 0; u < AS; ++u) {
+                if (R == 8) {
+                    idft8(v[u]);
+                } else if (R == 4) {
+                    idft4(v[u][0], v[u][2], v[u][4], v[u][6]);
+                    idft4(v[u][1], v[u][3], v[u][5], v[u][7]);
+                } else {
+                    idft2(v[u][0], v[u][4]);
+                    idft2(v[u][1], v[u][5]);
+                    idft2(v[u][2], v[u][6]);
+                    idft2(v[u][3], v[u][7]);
+                }
             }
             if (p < NP - 1) {
                 // Stockham placement of output m of butterfly b:
                 //   (b / ns) * ns * R + b % ns + m * ns, padded
 #pragma unroll
-                for (int q = 0; q < RQ; ++q) {
-                    const int b = t + q * TPA;
-                    float2* wp = dst + pad_idx((b / ns) * ns * R + (b & (ns - 1)));
+                for (int u = 0; u < AS; ++u) {
+                    float2* dst = (p % 2 == 0) ? pa[u] : pb[u];
 #pragma unroll
-                    for (int m = 0; m < R; ++m) {
-                        const int e = q + RQ * m;
-                        const int off = (ns == 1) ? m : m * (ns + ns / 8);
-                        wp[off] = make_float2(v[e].x, v[e].y);
+                    for (int q = 0; q < RQ; ++q) {
+                        const int b = t + q * TPA;
+                        float2* wp = dst + pad_idx((b / ns) * ns * R + (b & (ns - 1)));
+#pragma unroll
+                        for (int m = 0; m < R; ++m) {
+                            const int e = q + RQ * m;
+                            // base has no bits where m*ns has any: the padded
+                            // index splits into base and a constant
+                            wp[pad_slots(m * ns)] = make_float2(v[u][e].x, v[u][e].y);
+                        }
                     }
                 }
                 __syncthreads();
             }
         }
-        // after the last pass v[e] holds X[t + e*TPA]; e < 4 are the kept bins
+        // after the last pass v[u][e] holds X[t + e*TPA]; e < 4 are the kept bins
 
-        if constexpr (STATS) {
-            if (active) {
-                const long long row = A.row0 + a;
-                const int seg = (int)(row / A.seg_w);
-                if (seg != cur_seg) {
-                    if (cur_seg >= 0) {
 #pragma unroll
-                        for (int o = 0; o < 4; ++o) {
-                            double* d = A.acc + ((size_t)cur_seg * G::HALF + t + o * TPA) * 3;
-                            unsafeAtomicAdd(d, sx[o]);
-                            unsafeAtomicAdd(d + 1, sy[o]);
-                            unsafeAtomicAdd(d + 2, sxx[o]);
-                            sx[o] = sy[o] = sxx[o] = 0.0;
+        for (int u = 0; u < AS; ++u) {
+            const long long a = a0 + slot * AS + u;
+            const bool active = a < a_end;
+            if constexpr (STATS) {
+                if (active) {
+                    const long long row = A.row0 + a;
+                    const int seg = (int)(row / A.seg_w);
+                    if (seg != cur_seg[u]) {
+                        if (cur_seg[u] >= 0) {
+#pragma unroll
+                            for (int o = 0; o < 4; ++o) {
+                                double* d = A.acc + ((size_t)cur_seg[u] * G::HALF + t + o * TPA) * 3;
+                                unsafeAtomicAdd(d, sx[u][o]);
+                                unsafeAtomicAdd(d + 1, sy[u][o]);
+                                unsafeAtomicAdd(d + 2, sxx[u][o]);
+                                sx[u][o] = sy[u][o] = sxx[u][o] = 0.0;
+                            }
                         }
+                        cur_seg[u] = seg;
                     }
-                    cur_seg = seg;
-                }
-#pragma unroll
-                for (int o = 0; o < 4; ++o) {
-                    const double re = v[o].x, im = v[o].y;
-                    sx[o] += re;
-                    sy[o] += im;
-                    sxx[o] += re * re + im * im;
-                }
-            }
-        } else {
-            // ---- epilogue: FPN, magnitude, grayscale map, background
-            float m2[4];
-#pragma unroll
-            for (int o = 0; o < 4; ++o) {
-                const cf z = csub(v[o], fp[o]);
-                m2[o] = fmaf(z.x, z.x, z.y * z.y);
-            }
-            float res[4];
-            if (flags & F_LOG) {
-                if (flags & F_AUTOMM) {
-                    float val[4];
-                    float mn = INFINITY, mx = -INFINITY;
 #pragma unroll
                     for (int o = 0; o < 4; ++o) {
-                        val[o] = A.log_c * (__log2f(m2[o]) - A.log_h);
-                        mn = fminf(mn, val[o]);
-                        mx = fmaxf(mx, val[o]);
+                        const double re = v[u][o].x, im = v[u][o].y;
+                        sx[u][o] += re;
+                        sy[u][o] += im;
+                        sxx[u][o] += re * re + im * im;
                     }
-                    ascan_minmax<TPA, G::WG>(mn, mx, mm_part, mm_part + G::WG / 64);
-                    float range = mx - mn;
-                    if (range < 1e-6f) range = 1.0f;
-#pragma unroll
-                    for (int o = 0; o < 4; ++o) res[o] = A.coeff * ((val[o] - mn) / range + A.addend);
-                } else {
-#pragma unroll
-                    for (int o = 0; o < 4; ++o) res[o] = fmaf(A.ep_a, __log2f(m2[o]), A.ep_b);
                 }
             } else {
-#pragma unroll
-                for (int o = 0; o < 4; ++o) res[o] = fmaf(A.ep_a, __fsqrt_rn(m2[o]), A.ep_b);
-            }
-            if (active) {
-                float* dst = A.out + a * G::HALF + t;
+                // ---- epilogue: FPN, magnitude, grayscale map, background
+                float m2[4];
 #pragma unroll
                 for (int o = 0; o < 4; ++o) {
-                    const float r = (flags & F_BG) ? res[o] - bgv[o] : res[o];
-                    __builtin_nontemporal_store(r, dst + o * TPA);
+                    const cf z = csub(v[u][o], fp[o]);
+                    m2[o] = fmaf(z.x, z.x, z.y * z.y);
+                }
+                float res[4];
+                if (flags & F_LOG) {
+                    if (flags & F_AUTOMM) {
+                        float val[4];
+                        float mn = INFINITY, mx = -INFINITY;
+#pragma unroll
+                        for (int o = 0; o < 4; ++o) {
+                            val[o] = A.log_c * (__log2f(m2[o]) - A.log_h);
+                            mn = fminf(mn, val[o]);
+                            mx = fmaxf(mx, val[o]);
+                        }
+                        ascan_minmax<TPA, G::WG>(mn, mx, mm_part + 2 * u * NW, mm_part + (2 * u + 1) * NW);
+                        float range = mx - mn;
+                        if (range < 1e-6f) range = 1.0f;
+#pragma unroll
+                        for (int o = 0; o < 4; ++o) res[o] = A.coeff * ((val[o] - mn) / range + A.addend);
+                    } else {
+#pragma unroll
+                        for (int o = 0; o < 4; ++o) res[o] = fmaf(A.ep_a, __log2f(m2[o]), A.ep_b);
+                    }
+                } else {
+#pragma unroll
+                    for (int o = 0; o < 4; ++o) res[o] = fmaf(A.ep_a, __fsqrt_rn(m2[o]), A.ep_b);
+                }
+                if (active) {
+                    float* dst = A.out + a * G::HALF + t;
+#pragma unroll
+                    for (int o = 0; o < 4; ++o) {
+                        const float r = (flags & F_BG) ? res[o] - bgv[o] : res[o];
+                        __builtin_nontemporal_store(r, dst + o * TPA);
+                    }
                 }
             }
         }
     }
 
     if constexpr (STATS) {
-        if (cur_seg >= 0) {
 #pragma unroll
-            for (int o = 0; o < 4; ++o) {
-                double* d = A.acc + ((size_t)cur_seg * G::HALF + t + o * TPA) * 3;
-                unsafeAtomicAdd(d, sx[o]);
-                unsafeAtomicAdd(d + 1, sy[o]);
-                unsafeAtomicAdd(d + 2, sxx[o]);
+        for (int u = 0; u < AS; ++u) {
+            if (cur_seg[u] >= 0) {
+#pragma unroll
+                for (int o = 0; o < 4; ++o) {
+                    double* d = A.acc + ((size_t)cur_seg[u] * G::HALF + t + o * TPA) * 3;
+                    unsafeAtomicAdd(d, sx[u][o]);
+                    unsafeAtomicAdd(d + 1, sy[u][o]);
+                    unsafeAtomicAdd(d + 2, sxx[u][o]);
+                }
             }
         }
     }

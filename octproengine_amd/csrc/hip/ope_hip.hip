@@ -270,15 +270,23 @@ int validate_params(ope_hip_ctx* c, const ope_hip_params* p) {
 
 using KernelFn = void (*)(ChainArgs);
 
+// A-scans per lane.  Two (instruction-level parallelism) measured slower
+// than one on gfx950 for every mode: the extra VGPRs and LDS cost a wave of
+// occupancy per SIMD, which hides more latency than the second A-scan does.
+constexpr int kAscansPerLane = 1;
+
 template <int N, int MODE>
 KernelFn pick_t(int taps) {
+    constexpr int AS = kAscansPerLane;
     switch (taps) {
-        case 1: return chain_kernel<N, 1, MODE>;
-        case 2: return chain_kernel<N, 2, MODE>;
-        case 4: return chain_kernel<N, 4, MODE>;
-        default: return chain_kernel<N, 16, MODE>;
+        case 1: return chain_kernel<N, 1, MODE, AS>;
+        case 2: return chain_kernel<N, 2, MODE, AS>;
+        case 4: return chain_kernel<N, 4, MODE, AS>;
+        default: return chain_kernel<N, 16, MODE, AS>;
     }
 }
+
+int ascans_per_lane(int) { return kAscansPerLane; }
 
 template <int N>
 KernelFn pick_mode(int taps, int mode) {
@@ -349,7 +357,8 @@ int launch_chain(ope_hip_ctx* c, const ChainArgs& a, bool stats, long long work_
     int per_cu = 0;
     HIP_TRY(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(k), wg, 0));
     if (per_cu < 1) per_cu = 1;
-    long long groups = (work_ascans + apw - 1) / apw;
+    const int slots = apw * ascans_per_lane(mode);
+    long long groups = (work_ascans + slots - 1) / slots;
     long long grid = std::min<long long>(groups, (long long)per_cu * c->cu_count);
     if (grid < 1) return OPE_HIP_OK;
     hipLaunchKernelGGL(k, dim3((unsigned)grid), dim3(wg), 0, s, a);
