@@ -52,8 +52,43 @@ def build_hip(verbose_resources: bool = False) -> Path:
     return LIB_HIP
 
 
+CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wextra", "-Wno-unused-parameter",
+             "-ffp-contract=off", "-pthread"]
+
+
+def build_core() -> Path:
+    """libope.so: the ope:: C++ API and the HIP backend over the C ABI."""
+    srcs = [CSRC / "core" / f for f in ("iobuffer.cpp", "callback_manager.cpp",
+                                         "processorconfiguration.cpp", "processor.cpp")]
+    srcs.append(CSRC / "backends" / "hip_backend.cpp")
+    cmd = [CXX, *CXX_FLAGS, "-shared", "-DBUILD_OPE_LIBRARY", "-fvisibility=hidden",
+           f"-I{INCLUDE / 'ope'}", f"-I{INCLUDE}", *map(str, srcs), f"-L{PKG}", "-lope_hip",
+           "-Wl,-rpath,$ORIGIN", "-o", str(LIB_CORE)]
+    _run(cmd)
+    return LIB_CORE
+
+
+def python_module_path() -> Path:
+    suffix = sysconfig.get_config_var("EXT_SUFFIX") or ".so"
+    return PKG / f"octproengine{suffix}"
+
+
+def build_python() -> Path:
+    """The pybind11 module `octproengine`."""
+    import pybind11
+    out = python_module_path()
+    cmd = [CXX, *CXX_FLAGS, "-shared", "-fvisibility=hidden", f"-I{pybind11.get_include()}",
+           f"-I{sysconfig.get_paths()['include']}", f"-I{INCLUDE / 'ope'}",
+           str(CSRC / "python" / "bindings.cpp"), f"-L{PKG}", "-lope", "-Wl,-rpath,$ORIGIN",
+           "-o", str(out)]
+    _run(cmd)
+    return out
+
+
 def build_all(verbose_resources: bool = False) -> None:
     build_hip(verbose_resources)
+    build_core()
+    build_python()
 
 
 if __name__ == "__main__":

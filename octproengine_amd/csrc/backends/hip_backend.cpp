@@ -1,0 +1,309 @@
+// HIP backend: ProcessingBackend semantics of OCTproEngine's CPU backend
+// (the parity target, src/backends/cpu/cpu_backend.cpp) with the host-side
+// structure of its CUDA backend (src/backends/cuda/cuda_backend.cu):
+// a pool of pinned input buffers lent out by getNextAvailableInputBuffer(),
+// process() enqueues H2D + chain + D2H and returns, the input buffer returns
+// to the pool once its copy has landed, and the output callback fires from
+// the HIP runtime's callback thread when a frame's output is in host memory.
+//
+// Frames in flight each have their own device buffers and output buffer, so
+// consecutive frames never share device work memory (the CUDA backend shares
+// one set across its streams, cuda_backend.cu:77-79,448-452).
+#include "hip_backend.h"
+
+#include <cstdlib>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+
+#include "../../../include/ope_hip.h"
+#include "../core/iobuffer_access.h"
+
+namespace ope {
+
+namespace {
+
+void release_pinned(void* p) { ope_hip_host_free(p); }
+
+ope_hip_params to_params(const ProcessorConfiguration& c) {
+    ope_hip_params p{};
+    p.signal_length = c.dataParams.signalLength;
+    p.ascans_per_bscan = c.dataParams.ascansPerBscan;
+    p.bscans_per_buffer = c.dataParams.bscansPerBuffer;
+    p.input_type = static_cast<int>(c.dataParams.inputDataType);
+    p.resampling_enabled = c.resamplingParams.enabled;
+    p.interpolation = static_cast<int>(c.resamplingParams.interpolationMethod);
+    p.windowing_enabled = c.windowingParams.enabled;
+    p.dispersion_enabled = c.dispersionParams.enabled;
+    p.dc_removal_enabled = c.backgroundRemovalParams.enabled;
+    p.dc_window_size = c.backgroundRemovalParams.rollingAverageWindowSize;
+    p.log_scaling = c.postProcessingParams.logScaling;
+    p.grayscale_min = c.postProcessingParams.grayscaleMin;
+    p.grayscale_max = c.postProcessingParams.grayscaleMax;
+    p.multiplicator = c.postProcessingParams.multiplicator;
+    p.addend = c.postProcessingParams.addend;
+    p.fpn_removal = c.postProcessingParams.fixedPatternNoiseRemoval;
+    p.fpn_bscan_count = c.postProcessingParams.fixedPatternNoiseBscanCount;
+    p.fpn_continuous = c.postProcessingParams.continuousFixedPatternNoiseDetermination;
+    p.bg_removal = c.postProcessingParams.backgroundRemoval;
+    p.bg_weight = c.postProcessingParams.backgroundWeight;
+    p.bg_offset = c.postProcessingParams.backgroundOffset;
+    return p;
+}
+
+int device_from_env() {
+    const char* s = std::getenv("OPE_HIP_DEVICE");
+    return s ? std::atoi(s) : 0;
+}
+
+}  // namespace
+
+HipBackend::HipBackend() = default;
+
+HipBackend::~HipBackend() { cleanup(); }
+
+void HipBackend::check(int rc, const char* what) const {
+    if (rc == OPE_HIP_OK) return;
+    std::string msg = std::string("HIP backend: ") + what + ": " + ope_hip_last_error(ctx_);
+    if (rc == OPE_HIP_EINVAL) throw std::invalid_argument(msg);
+    throw std::runtime_error(msg);
+}
+
+void HipBackend::requireContext(const char* what) const {
+    if (!ctx_) throw std::runtime_error(std::string("HIP backend not initialized (") + what + ")");
+}
+
+void HipBackend::initialize(const ProcessorConfiguration& config) {
+    if (ctx_) cleanup();
+    config_ = config;
+    const ope_hip_params params = to_params(config_);
+    ope_hip_ctx* ctx = nullptr;
+    int rc = ope_hip_create(device_from_env(), &params, kSlots, &ctx);
+    if (rc != OPE_HIP_OK) {
+        std::string msg = std::string("HIP backend initialization failed: ") + ope_hip_last_error(nullptr);
+        if (rc == OPE_HIP_EINVAL || rc == OPE_HIP_EUNSUP) throw std::invalid_argument(msg);
+        throw std::runtime_error(msg);
+    }
+    ctx_ = ctx;
+
+    const size_t n = static_cast<size_t>(config_.dataParams.signalLength);
+    const size_t ascans = static_cast<size_t>(config_.dataParams.ascansPerBscan) * config_.dataParams.bscansPerBuffer;
+    const size_t samples = std::max(static_cast<size_t>(config_.dataParams.samplesPerBuffer), n * ascans);
+    const size_t inBytes = samples * config_.dataParams.getBytesPerSample();
+    const size_t outBytes = (samples / 2) * sizeof(float);
+    auto pinned = [&](size_t bytes, DataType type) {
+        std::unique_ptr<IOBuffer> b(new IOBuffer());
+        void* p = nullptr;
+        if (ope_hip_host_alloc(bytes, &p) != OPE_HIP_OK)
+            throw std::runtime_error("HIP backend: pinned host allocation failed");
+        detail::IOBufferAccess::adopt(*b, p, bytes, release_pinned);
+        b->setDataType(type);
+        return b;
+    };
+    for (int i = 0; i < kInputBuffers; ++i) inputs_.push_back(pinned(inBytes, config_.dataParams.inputDataType));
+    for (int i = 0; i < kSlots; ++i) outputs_.push_back(pinned(outBytes, DataType::FLOAT32));
+    {
+        std::lock_guard<std::mutex> lk(freeMu_);
+        free_.clear();
+        for (auto& b : inputs_) free_.push_back(b.get());
+    }
+    for (int i = 0; i < kSlots; ++i) slots_[i] = SlotCtx{this, outputs_[i].get()};
+    for (int i = 0; i < kInputBuffers; ++i) inputCtx_[i] = InputCtx{this, inputs_[i].get()};
+    nextSlot_ = 0;
+    if (pendingFpnRequest_) check(ope_hip_request_fpn(ctx_), "requestFixedPatternNoiseDetermination");
+    if (pendingBgRequest_) check(ope_hip_request_bg_recording(ctx_), "requestPostProcessBackgroundRecording");
+    pendingFpnRequest_ = pendingBgRequest_ = false;
+}
+
+void HipBackend::cleanup() {
+    if (ctx_) {
+        ope_hip_synchronize(ctx_);
+        ope_hip_destroy(ctx_);
+        ctx_ = nullptr;
+    }
+    {
+        std::lock_guard<std::mutex> lk(freeMu_);
+        free_.clear();
+    }
+    inputs_.clear();
+    outputs_.clear();
+}
+
+void HipBackend::setOutputCallback(std::function<void(const IOBuffer&)> callback) {
+    std::lock_guard<std::mutex> lk(callbackMu_);
+    callback_ = std::move(callback);
+}
+
+void HipBackend::onInputFree(void* user) {
+    InputCtx* c = static_cast<InputCtx*>(user);
+    HipBackend* self = c->self;
+    IOBuffer* in = c->input;
+    {
+        std::lock_guard<std::mutex> lk(self->freeMu_);
+        self->free_.push_back(in);
+    }
+    self->freeCv_.notify_one();
+}
+
+void HipBackend::onFrameDone(void* user, float*) {
+    SlotCtx* s = static_cast<SlotCtx*>(user);
+    std::function<void(const IOBuffer&)> cb;
+    {
+        std::lock_guard<std::mutex> lk(s->self->callbackMu_);
+        cb = s->self->callback_;
+    }
+    if (cb) cb(*s->output);
+}
+
+void HipBackend::process(IOBuffer& input) {
+    requireContext("process");
+    const size_t need = static_cast<size_t>(config_.dataParams.signalLength) * config_.dataParams.ascansPerBscan *
+                        config_.dataParams.bscansPerBuffer * config_.dataParams.getBytesPerSample();
+    if (!input.getDataPointer() || input.getSizeInBytes() < need)
+        throw std::invalid_argument("HIP backend: input buffer is smaller than one frame");
+    // Pool buffers go back to the pool once their copy has landed; a
+    // caller-owned buffer is not adopted into the pool.
+    InputCtx* ic = nullptr;
+    for (int i = 0; i < kInputBuffers && i < static_cast<int>(inputs_.size()); ++i)
+        if (inputs_[i].get() == &input) ic = &inputCtx_[i];
+    SlotCtx& s = slots_[nextSlot_];
+    nextSlot_ = (nextSlot_ + 1) % kSlots;
+    check(ope_hip_submit_host(ctx_, input.getDataPointer(), static_cast<float*>(s.output->getDataPointer()),
+                              ic ? &HipBackend::onInputFree : nullptr, ic, &HipBackend::onFrameDone, &s),
+          "process");
+}
+
+void HipBackend::updateConfig(const ProcessorConfiguration& config) {
+    config_ = config;
+    if (!ctx_) return;
+    const ope_hip_params p = to_params(config_);
+    check(ope_hip_set_params(ctx_, &p), "updateConfig");
+}
+
+void HipBackend::updateResamplingCurve(const float* curve, size_t length) {
+    if (!curve && length > 0) throw std::runtime_error("Invalid resampling curve pointer");
+    requireContext("updateResamplingCurve");
+    check(ope_hip_set_resampling_curve(ctx_, curve, length), "updateResamplingCurve");
+}
+
+void HipBackend::updateDispersionCurve(const float* curve, size_t length) {
+    requireContext("updateDispersionCurve");
+    check(ope_hip_set_dispersion_curve(ctx_, curve, length), "updateDispersionCurve");
+}
+
+void HipBackend::updateWindowCurve(const float* curve, size_t length) {
+    if (!curve && length > 0) throw std::runtime_error("Invalid window curve pointer");
+    requireContext("updateWindowCurve");
+    check(ope_hip_set_window_curve(ctx_, curve, length), "updateWindowCurve");
+}
+
+IOBuffer& HipBackend::getInputBuffer(int index) {
+    if (index < 0 || index >= static_cast<int>(inputs_.size())) throw std::runtime_error("Buffer index out of range");
+    return *inputs_[index];
+}
+
+IOBuffer& HipBackend::getNextAvailableInputBuffer() {
+    requireContext("getNextAvailableInputBuffer");
+    std::unique_lock<std::mutex> lk(freeMu_);
+    freeCv_.wait(lk, [this] { return !free_.empty(); });
+    IOBuffer* b = free_.front();
+    free_.pop_front();
+    return *b;
+}
+
+int HipBackend::getNumInputBuffers() const { return static_cast<int>(inputs_.size()); }
+
+void HipBackend::requestPostProcessBackgroundRecording() {
+    if (!ctx_) {
+        pendingBgRequest_ = true;
+        return;
+    }
+    check(ope_hip_request_bg_recording(ctx_), "requestPostProcessBackgroundRecording");
+}
+
+void HipBackend::setPostProcessBackgroundProfile(const float* background, size_t length) {
+    if (!background) throw std::invalid_argument("Background curve data is null");
+    requireContext("setPostProcessBackgroundProfile");
+    check(ope_hip_set_bg_profile(ctx_, background, length), "setPostProcessBackgroundProfile");
+}
+
+const std::vector<float>& HipBackend::getPostProcessBackgroundProfile() const {
+    bgCache_.clear();
+    if (!ctx_) return bgCache_;
+    size_t len = 0;
+    check(ope_hip_get_bg_profile(ctx_, nullptr, 0, &len), "getPostProcessBackgroundProfile");
+    bgCache_.resize(len);
+    if (len) check(ope_hip_get_bg_profile(ctx_, bgCache_.data(), len, &len), "getPostProcessBackgroundProfile");
+    return bgCache_;
+}
+
+void HipBackend::requestFixedPatternNoiseDetermination() {
+    if (!ctx_) {
+        pendingFpnRequest_ = true;
+        return;
+    }
+    check(ope_hip_request_fpn(ctx_), "requestFixedPatternNoiseDetermination");
+}
+
+void HipBackend::setFixedPatternNoiseProfile(const float* profileInterleaved, size_t complexPairs) {
+    if (!profileInterleaved && complexPairs > 0)
+        throw std::invalid_argument("Invalid fixed-pattern noise profile pointer");
+    requireContext("setFixedPatternNoiseProfile");
+    check(ope_hip_set_fpn_profile(ctx_, profileInterleaved, complexPairs), "setFixedPatternNoiseProfile");
+}
+
+const std::vector<float>& HipBackend::getFixedPatternNoiseProfile() const {
+    fpnCache_.clear();
+    if (!ctx_) return fpnCache_;
+    size_t pairs = 0;
+    check(ope_hip_get_fpn_profile(ctx_, nullptr, 0, &pairs), "getFixedPatternNoiseProfile");
+    fpnCache_.resize(2 * pairs);
+    if (pairs) check(ope_hip_get_fpn_profile(ctx_, fpnCache_.data(), pairs, &pairs), "getFixedPatternNoiseProfile");
+    return fpnCache_;
+}
+
+// ---- individual operations
+//
+// getMinimumVarianceMean runs on the GPU.  The other operations return empty
+// vectors, as the CUDA backend's do (cuda_backend.cu:1133-1296): they are
+// per-stage debugging hooks of the reference, not part of the processing
+// path.
+
+std::vector<float> HipBackend::getMinimumVarianceMean(const float* input, int width, int height, int segments) {
+    if (!input) return {};
+    requireContext("getMinimumVarianceMean");
+    std::vector<float> out(static_cast<size_t>(std::max(width, 0)) * 2);
+    check(ope_hip_op_min_variance_mean(ctx_, input, width, height, segments, out.data()), "getMinimumVarianceMean");
+    return out;
+}
+
+std::vector<float> HipBackend::convertInput(const void*, IOBuffer::DataType, int, int, bool) { return {}; }
+std::vector<float> HipBackend::rollingAverageBackgroundRemoval(const float*, int, int, int) { return {}; }
+std::vector<float> HipBackend::kLinearization(const float*, const float*, InterpolationMethod, int, int) { return {}; }
+std::vector<float> HipBackend::windowing(const float*, const float*, int, int) { return {}; }
+std::vector<float> HipBackend::dispersionCompensation(const float*, const float*, int, int) { return {}; }
+std::vector<float> HipBackend::kLinearizationAndWindowing(const float*, const float*, const float*,
+                                                          InterpolationMethod, int, int) {
+    return {};
+}
+std::vector<float> HipBackend::kLinearizationAndWindowingAndDispersion(const float*, const float*, const float*,
+                                                                       const float*, InterpolationMethod, int, int) {
+    return {};
+}
+std::vector<float> HipBackend::dispersionCompensationAndWindowing(const float*, const float*, const float*, int,
+                                                                  int) {
+    return {};
+}
+std::vector<float> HipBackend::fft(const float*, int, int) { return {}; }
+std::vector<float> HipBackend::ifft(const float*, int, int) { return {}; }
+std::vector<float> HipBackend::fixedPatternNoiseRemoval(const float*, const float*, int, int) { return {}; }
+std::vector<float> HipBackend::postProcessTruncate(const float*, bool, float, float, float, float, int, int) {
+    return {};
+}
+std::vector<float> HipBackend::bscanFlip(const float*, int, int, int) { return {}; }
+std::vector<float> HipBackend::sinusoidalScanCorrection(const float*, const float*, int, int, int) { return {}; }
+std::vector<float> HipBackend::postProcessBackgroundSubtraction(const float*, const float*, float, float, int, int) {
+    return {};
+}
+
+}  // namespace ope

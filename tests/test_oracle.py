@@ -1,0 +1,257 @@
+"""Pins the FP64 oracle (oracle/chain64.py) and the float32 C restatement
+(oracle/cpu_chain.c) before anything is checked against them.
+
+Every check is independent of the oracle's own formulas: closed forms,
+brute force on tiny inputs, invariants, and golden curves written by the
+reference's own code.  The `planted` tests verify that a plausible mistake
+(sign, index, dropped term) in an oracle function makes its check fail.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import math
+
+import numpy as np
+import pytest
+
+from oracle import chain64 as O
+from tests.helpers import CpuParams, Tables, _fp, cpu32, cpu_lib, db_error, db_params
+
+RNG = np.random.default_rng(1234)
+
+
+# ------------------------------------------------------------------ DFT
+
+def brute_idft(s):
+    n = len(s)
+    j = np.arange(n)
+    return np.array([np.sum(s * np.exp(2j * np.pi * j * k / n)) for k in range(n)])
+
+
+def check_ifft(fn):
+    for n in (8, 16, 64, 100):
+        s = RNG.normal(size=n) + 1j * RNG.normal(size=n)
+        X = fn(s)
+        assert np.allclose(X, brute_idft(s), atol=1e-9 * n)
+        # Parseval for the unnormalised transform: sum |X|^2 = N sum |s|^2
+        assert math.isclose(np.sum(np.abs(X) ** 2), n * np.sum(np.abs(s) ** 2), rel_tol=1e-12)
+    # a cosine at bin z peaks at k = z and N - z
+    n, z = 256, 37
+    X = fn(np.cos(2 * np.pi * z * np.arange(n) / n))
+    assert set(np.argsort(np.abs(X))[-2:]) == {z, n - z}
+
+
+def test_ifft_definition():
+    check_ifft(O.ifft_unnormalized)
+
+
+def test_planted_ifft_sign_error_is_caught():
+    with pytest.raises(AssertionError):
+        check_ifft(lambda s: np.fft.fft(s))  # forward sign instead of backward
+
+
+# ------------------------------------------------------------------ interpolation
+
+def check_linear(fn):
+    n = 64
+    x = 3.0 * np.arange(n) - 7.0  # affine data
+    r = np.linspace(0.0, n - 2.0, 200).astype(np.float32)
+    assert np.allclose(fn(x, r), 3.0 * r.astype(np.float64) - 7.0, atol=1e-12)
+    # equals numpy.interp on arbitrary data inside the range
+    y = RNG.normal(size=n)
+    assert np.allclose(fn(y, r), np.interp(r.astype(np.float64), np.arange(n), y), atol=1e-12)
+
+
+def test_linear_interpolation():
+    check_linear(O.klin_linear)
+    # clamping at the upper end reproduces the last sample
+    x = RNG.normal(size=32)
+    assert O.klin_linear(x, np.array([31.0, 40.0], np.float32))[0] == x[31]
+
+
+def test_planted_linear_index_error_is_caught():
+    def bad(x, curve):
+        r = np.asarray(curve, np.float64)
+        n1 = np.minimum(np.trunc(r).astype(int), len(x) - 1)
+        n2 = np.minimum(n1 + 1, len(x) - 1)
+        return x[n2] + (x[n2] - x[n1]) * (r - n1)  # wrong base sample
+    with pytest.raises(AssertionError):
+        check_linear(bad)
+
+
+def check_cubic(fn):
+    n = 64
+    j = np.arange(n, dtype=np.float64)
+    q = 0.5 * j * j - 3.0 * j + 2.0  # Catmull-Rom reproduces quadratics exactly
+    r = np.linspace(1.0, n - 3.0, 301).astype(np.float32)
+    rr = r.astype(np.float64)
+    assert np.allclose(fn(q, r), 0.5 * rr * rr - 3.0 * rr + 2.0, atol=1e-9)
+    y = RNG.normal(size=n)
+    ints = np.arange(1, n - 3, dtype=np.float32)
+    assert np.allclose(fn(y, ints), y[1:n - 3], atol=1e-15)  # interpolating
+
+
+def test_cubic_interpolation():
+    check_cubic(O.klin_cubic)
+    # the reference's n0 = |n1 - 1| at n1 = 0 (cpu_kernels.tpp:127)
+    y = RNG.normal(size=16)
+    t = 0.25
+    got = O.klin_cubic(y, np.array([t], np.float32))[0]
+    assert math.isclose(got, O.cubic_hermite(y[1], y[0], y[1], y[2], np.float64(np.float32(t))), rel_tol=1e-14)
+
+
+def test_planted_cubic_dropped_term_is_caught():
+    def bad(x, curve):
+        r = np.asarray(curve, np.float32).astype(np.float64)
+        n1 = np.trunc(r).astype(int)
+        y0, y1, y2, y3 = x[np.abs(n1 - 1)], x[n1], x[n1 + 1], x[np.minimum(n1 + 2, len(x) - 1)]
+        t = r - n1
+        a = -y0 + 3.0 * (y1 - y2) + y3
+        c = -y0 + y2  # the b t^2 term dropped
+        return 0.5 * t * (a * t * t + c) + y1
+    with pytest.raises(AssertionError):
+        check_cubic(bad)
+
+
+def check_lanczos(fn):
+    # sinc(d) sinc(d/8) in closed form, brute force
+    d = np.array([0.0, 1e-6, 0.3, 1.0, 2.5, 7.9, 8.0, 9.0, -3.2])
+    want = []
+    for v in d:
+        if abs(v) < 1e-5:
+            want.append(1.0)
+        elif abs(v) >= 8:
+            want.append(0.0)
+        else:
+            want.append(math.sin(math.pi * v) / (math.pi * v) * math.sin(math.pi * v / 8) / (math.pi * v / 8))
+    assert np.allclose(fn(d), want, atol=1e-15)
+
+
+def test_lanczos():
+    check_lanczos(O.lanczos_kernel)
+    # interpolating: integer positions return the samples
+    y = RNG.normal(size=64)
+    ints = np.arange(8, 56, dtype=np.float32)
+    assert np.allclose(O.klin_lanczos(y, ints), y[8:56], atol=1e-12)
+    # a constant signal stays (nearly) constant: the kernel sums to ~1
+    got = O.klin_lanczos(np.ones(64), np.linspace(10, 50, 77).astype(np.float32))
+    assert np.all(np.abs(got - 1.0) < 0.05)
+
+
+def test_planted_lanczos_wrong_scale_is_caught():
+    with pytest.raises(AssertionError):
+        check_lanczos(lambda d: O.lanczos_kernel(np.asarray(d) * 1.01))
+
+
+# ------------------------------------------------------------------ DC removal, FPN, scaling
+
+def brute_dc(x, w):
+    n = len(x)
+    out = np.empty(n)
+    for i in range(n):
+        lo = max(i - (w - 1), 0) if w >= 1 else 0
+        hi = min(i + w, n - 1) if w >= 1 else i
+        out[i] = x[i] - np.mean(x[lo:hi + 1])
+    return out
+
+
+def test_rolling_average_dc():
+    x = RNG.normal(size=50) + 10
+    for w in (0, 1, 3, 7, 49, 80):
+        assert np.allclose(O.rolling_average_dc(x, w), brute_dc(x, w), atol=1e-12)
+
+
+def test_planted_dc_window_off_by_one_is_caught():
+    x = RNG.normal(size=50) + 10
+    bad = brute_dc(x, 8)  # the window of a different size
+    assert not np.allclose(O.rolling_average_dc(x, 7), bad, atol=1e-9)
+
+
+def brute_mvm(rows, segments):
+    h, w = rows.shape
+    sw = h // segments
+    out = np.zeros(2 * w, np.float32)
+    for c in range(w):
+        best = None
+        for s in range(segments):
+            seg = rows[s * sw:(s + 1) * sw, c]
+            var = np.mean(np.abs(seg) ** 2) - np.abs(np.mean(seg)) ** 2
+            if best is None or var < best[0]:
+                best = (var, np.mean(seg))
+        out[2 * c], out[2 * c + 1] = best[1].real, best[1].imag
+    return out
+
+
+def test_min_variance_mean_brute_force():
+    # well-separated segment variances: the float32/float64 mixture of the
+    # reference picks the same segment as exact arithmetic
+    h, w, segs = 40, 6, 8
+    scale = np.repeat(np.array([5.0, 1.0, 3.0, 0.2, 4.0, 2.0, 6.0, 7.0]), h // segs)[:, None]
+    rows = (RNG.normal(size=(h, w)) + 1j * RNG.normal(size=(h, w))) * scale + (2 + 1j)
+    rows = rows.real.astype(np.float32) + 1j * rows.imag.astype(np.float32)
+    got = O.min_variance_mean(rows, segs)
+    assert np.allclose(got, brute_mvm(rows, segs), rtol=1e-6, atol=1e-6)
+    # and the C restatement agrees bit for bit with the oracle
+    lib = cpu_lib()
+    inter = np.empty((h, w, 2), np.float32)
+    inter[..., 0], inter[..., 1] = rows.real, rows.imag
+    out = np.zeros(2 * w, np.float32)
+    lib.ope_cpu_min_variance_mean(inter.ctypes.data, w, h, segs, out.ctypes.data)
+    assert np.array_equal(out, got)
+
+
+def test_log_and_linear_scaling_closed_forms():
+    X = np.array([[3 + 4j, 0.0, 1.0, 2.0, 0, 0, 0, 0]])
+    half = 4
+    log = O.log_scale_and_truncate(X, 2.0, 10.0, 60.0, 0.5, False)
+    with np.errstate(divide="ignore"):
+        v = 10 * np.log10(np.array([25.0, 0.0, 1.0, 4.0]) / half)
+    assert np.allclose(log[0], 2.0 * ((v - 10.0) / 50.0 + 0.5))
+    assert log[0, 1] == -np.inf  # zero spectrum gives -inf, as the reference
+    lin = O.linear_scale_and_truncate(X, 1.5, 0.1, 0.6, 0.0)
+    assert np.allclose(lin[0], 1.5 * ((np.array([5.0, 0, 1, 2]) / half - 0.1) / 0.5))
+    # guard: range < 1e-6 -> 1; auto min/max per A-scan when min == max
+    Y = np.array([[1.0, 2.0, 4.0, 8.0, 0, 0, 0, 0]], complex)
+    auto = O.log_scale_and_truncate(Y, 1.0, 5.0, 5.0, 0.0, True)
+    assert np.isclose(auto.min(), 0.0) and np.isclose(auto.max(), 1.0)
+
+
+# ------------------------------------------------------------------ curves vs the reference's own code
+
+def _fixtures():
+    from pathlib import Path
+    return np.load(Path(__file__).parent / "golden" / "reference_curves.npz")
+
+
+def test_oracle_curves_match_reference_generator():
+    F = _fixtures()
+    for n in F["lengths"]:
+        for i, c in enumerate(F["params_resample"]):
+            assert np.array_equal(O.generate_resampling_curve(int(n), c), F[f"resample_n{n}_{i}"])
+        for i, d in enumerate(F["params_dispersion"]):
+            assert np.array_equal(O.generate_dispersion_phase(int(n), d), F[f"dispersion_n{n}_{i}"])
+        for i, (t, c, f) in enumerate(F["params_window"]):
+            got = O.generate_window(int(n), int(t), c, f)
+            want = F[f"window_n{n}_{i}"]
+            if int(t) == O.GAUSS:  # expf rounding: within 1 ulp
+                ulp = np.abs(got.view(np.int32).astype(np.int64) - want.view(np.int32))
+                assert ulp.max() <= 1
+            else:
+                assert np.array_equal(got, want)
+
+
+# ------------------------------------------------------------------ C restatement vs FP64
+
+@pytest.mark.parametrize("method", [O.LINEAR, O.CUBIC, O.LANCZOS])
+def test_cpu32_restatement_matches_fp64_oracle(method):
+    from octproengine_amd import synthetic as S
+    n, a = 512, 12
+    p = db_params(O.ChainParams(signal_length=n, ascans_per_bscan=a, resampling=True, interpolation=method,
+                                windowing=True, dispersion=True))
+    raw = S.fringes(n, a, seed=2)
+    t = Tables(S.resampling_curve(n), O.generate_window(n, O.HANN, 0.5, 0.95),
+               O.phase_to_phasor(S.dispersion_phase(n, 2e-5, -1e-8)))
+    ref = O.process_frame(raw, p, t.curve, t.window, t.phasor)
+    got = cpu32(p, raw, t)
+    assert db_error(got, ref) <= 1e-3
