@@ -76,6 +76,22 @@ def cos_sin(phase):
     return out
 
 
+TRAFFIC_SUMMARY = ROOT / "profiles" / "chain_kernel_2048x16384_u16_cubic_fpn_disp_log.hbm_traffic.md"
+
+
+def measured_traffic():
+    """HBM bytes per launch of the chain kernel on this workload, from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (calibrated against
+    a copy kernel of known size); None if the summary is absent."""
+    try:
+        for line in TRAFFIC_SUMMARY.read_text().splitlines():
+            if line.startswith("traffic_bytes_per_launch:"):
+                return int(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(cfg, seconds: float, threads: int):
     """The oracle's float32 C restatement of the reference CPU backend on
     frames of the same workload, single worker thread like the reference
@@ -114,14 +130,18 @@ def main():
     ap.add_argument("--ring", type=int, default=8, help="distinct frames cycled through")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--host-path", action="store_true",
+                    help="also time host frames through ope_hip_submit_host (pinned host buffers, "
+                         "H2D + chain + D2H): the PCIe-inclusive rate, reported beside the value")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 256 MiB read + 256 MiB write streaming kernel, the "
                          "reference point for the FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(A-scans are independent; see the sharding notes in DESIGN.md).", file=sys.stderr)
+        print("bench.py measures one GPU only: there is no multi-GPU path. A-scans are "
+              "independent, so frames would be split across GPUs by B-scan with no "
+              "communication.", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
@@ -175,6 +195,36 @@ def main():
     torch.cuda.synchronize()
     batched = 4 * frame * reps / (ev0.elapsed_time(ev1) / 1e3)
 
+    host_rate = None
+    if args.host_path:
+        import threading
+        in_bytes = frame * n * 2
+        out_bytes = frame * (n // 2) * 4
+        hin, hout = [], []
+        for i in range(2):
+            p_in, p_out = ctypes.c_void_p(), ctypes.c_void_p()
+            assert capi.lib.ope_hip_host_alloc(in_bytes, ctypes.byref(p_in)) == 0
+            assert capi.lib.ope_hip_host_alloc(out_bytes, ctypes.byref(p_out)) == 0
+            host = np.ctypeslib.as_array((ctypes.c_uint16 * (frame * n)).from_address(p_in.value))
+            host[:] = ins[i].cpu().numpy().view(np.uint16).reshape(-1)
+            hin.append(p_in.value)
+            hout.append(p_out.value)
+        done = threading.Semaphore(0)
+        frames_host = max(8, args.steps // 2)
+        for i in range(2):  # warm-up
+            ctx.submit_host(hin[i % 2], hout[i % 2], on_done=done.release)
+        for _ in range(2):
+            done.acquire()
+        t0 = time.perf_counter()
+        for i in range(frames_host):
+            ctx.submit_host(hin[i % 2], hout[i % 2], on_done=done.release)
+        for _ in range(frames_host):
+            done.acquire()
+        host_rate = frame * frames_host / (time.perf_counter() - t0)
+        ctx.synchronize()
+        for ptr in hin + hout:
+            capi.lib.ope_hip_host_free(ctypes.c_void_p(ptr))
+
     if args.copy_calibration:
         src = torch.ones(64 << 20, dtype=torch.float32, device=dev)
         dst = torch.empty_like(src)
@@ -190,6 +240,8 @@ def main():
     bytes_per_ascan = n * 2 + (n // 2) * 4
     table_bytes = 24 * n
     algo_bytes = frame * bytes_per_ascan + table_bytes
+    # one chain launch per step in steady state: the timed region's average
+    # per step is the kernel's average launch duration (launch gaps included)
     achieved = algo_bytes / (ms_per_step / 1e3) / 1e9
 
     result = {
@@ -209,9 +261,12 @@ def main():
                    "ascans_per_frame": frame, "input": "uint16", "frames_in_ring": args.ring,
                    "wall_s_timed_region": round(wall, 4)},
         "batched_4_frames_per_launch": {"value": round(batched, 1), "unit": "A-scans/s"},
+        "host_buffers_pcie_inclusive": (None if host_rate is None else
+                                        {"value": round(host_rate, 1), "unit": "A-scans/s",
+                                         "path": "ope_hip_submit_host, 2 pinned frames in flight"}),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "chain_kernel<2048, 4 taps>", "bytes_per_launch": algo_bytes},
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(),
+                     "kernel": "chain_kernel<2048, 4, 0, 2, 1>", "bytes_per_launch": algo_bytes},
     }
     if not args.no_cpu_baseline:
         v1, done1, dt1 = cpu_baseline(cfg, args.cpu_seconds, 1)
@@ -222,6 +277,15 @@ def main():
                       f"with a self-written radix-4 FFT (FFTW absent); host nproc={os.cpu_count()}",
         }
         result["gpu_over_cpu"] = round(ascans_per_s / v1, 1)
+        # all host cores of this process's share (the A-scan loop split across
+        # threads), so the ratio is also shown against a fully used host
+        nt = max(1, min(16, len(os.sched_getaffinity(0))))
+        vn, donen, dtn = cpu_baseline(cfg, args.cpu_seconds / 2, nt)
+        result["cpu_baseline_all_cores"] = {
+            "value": round(vn, 1), "unit": "A-scans/s", "cores": nt, "kind": "port",
+            "sample": f"{donen} A-scans of the same chain in {dtn:.1f} s, A-scan loop split over {nt} threads",
+        }
+        result["gpu_over_cpu_all_cores"] = round(ascans_per_s / vn, 1)
     print(json.dumps(result), flush=True)
     ctx.close()
 

@@ -53,6 +53,10 @@ SIGNATURES = {
 }
 
 
+FRAME_CB = ctypes.CFUNCTYPE(None, c_void_p, c_void_p)
+INPUT_CB = ctypes.CFUNCTYPE(None, c_void_p)
+
+
 class Params(Structure):
     """ope_hip_params (include/ope_hip.h)."""
     _fields_ = [
@@ -212,6 +216,16 @@ class Context:
         self._check(lib.ope_hip_op_min_variance_mean(self._h, inter.ctypes.data, w, h, segments,
                                                      out.ctypes.data))
         return out
+
+    def submit_host(self, host_in: int, host_out: int, on_done=None, on_input_free=None) -> None:
+        """Host frame: H2D, chain, D2H on the context's streams; returns once
+        enqueued.  Callbacks run on a HIP runtime thread."""
+        done = FRAME_CB(lambda user, out: on_done()) if on_done else FRAME_CB()
+        free = INPUT_CB(lambda user: on_input_free()) if on_input_free else INPUT_CB()
+        self._keep = getattr(self, "_keep", [])[-16:] + [done, free]  # alive until fired
+        self._check(lib.ope_hip_submit_host(self._h, c_void_p(host_in), c_void_p(host_out),
+                                            ctypes.cast(free, c_void_p), None,
+                                            ctypes.cast(done, c_void_p), None))
 
     def synchronize(self) -> None:
         self._check(lib.ope_hip_synchronize(self._h))

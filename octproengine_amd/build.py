@@ -29,13 +29,15 @@ ARCH = "gfx950"
 
 # Packed-FP32 VALU forms are kept out of the generated code.
 HIP_FLAGS = [
-    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
+    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-munsafe-fp-atomics", "-Wall", "-Wno-unused-function",
 ]
+CHAIN_LENGTHS = (256, 512, 1024, 2048, 4096)
 
 LIB_HIP = PKG / "libope_hip.so"
 LIB_CORE = PKG / "libope.so"
+OBJ_DIR = PKG / "_obj"
 
 
 def _run(cmd: list[str]) -> None:
@@ -43,12 +45,41 @@ def _run(cmd: list[str]) -> None:
     subprocess.run(cmd, check=True)
 
 
+def _run_parallel(cmds: list[list[str]], jobs: int) -> None:
+    """Runs the commands, at most `jobs` at a time; raises if any fails."""
+    pending = list(cmds)
+    running: list[tuple[list[str], subprocess.Popen]] = []
+    failed = []
+    while pending or running:
+        while pending and len(running) < jobs:
+            cmd = pending.pop(0)
+            print("+", " ".join(shlex.quote(c) for c in cmd), flush=True)
+            running.append((cmd, subprocess.Popen(cmd)))
+        cmd, proc = running.pop(0)
+        if proc.wait() != 0:
+            failed.append(cmd)
+    if failed:
+        raise subprocess.CalledProcessError(1, failed[0])
+
+
 def build_hip(verbose_resources: bool = False) -> Path:
-    src = CSRC / "hip" / "ope_hip.hip"
-    cmd = [HIPCC, *HIP_FLAGS, f"-I{INCLUDE}", str(src), "-o", str(LIB_HIP)]
-    if verbose_resources:
-        cmd.insert(1, "-Rpass-analysis=kernel-resource-usage")
-    _run(cmd)
+    """libope_hip.so: the C ABI host code plus one object per signal length
+    holding that length's chain-kernel instantiations, compiled in parallel."""
+    hip_dir = CSRC / "hip"
+    OBJ_DIR.mkdir(exist_ok=True)
+    extra = ["-Rpass-analysis=kernel-resource-usage"] if verbose_resources else []
+    objs, cmds = [], []
+    host_obj = OBJ_DIR / "ope_hip.o"
+    cmds.append([HIPCC, *HIP_FLAGS, *extra, f"-I{INCLUDE}", "-c", str(hip_dir / "ope_hip.hip"), "-o", str(host_obj)])
+    objs.append(host_obj)
+    for n in CHAIN_LENGTHS:
+        obj = OBJ_DIR / f"chain_n{n}.o"
+        cmds.append([HIPCC, *HIP_FLAGS, *extra, f"-I{INCLUDE}", f"-DOPE_CHAIN_N={n}", "-c",
+                     str(hip_dir / "chain_instances.hip"), "-o", str(obj)])
+        objs.append(obj)
+    jobs = max(1, min(len(cmds), int(os.environ.get("MAX_JOBS", os.cpu_count() or 1))))
+    _run_parallel(cmds, jobs)
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *map(str, objs), "-o", str(LIB_HIP)])
     return LIB_HIP
 
 

@@ -180,9 +180,14 @@ void HipBackend::updateConfig(const ProcessorConfiguration& config) {
     check(ope_hip_set_params(ctx_, &p), "updateConfig");
 }
 
+// A curve whose length differs from the initialised signal length arrives
+// when the Processor changes dimensions; it is ignored (as the CUDA backend
+// does, cuda_backend.cu:785-787,816-818) because the re-initialisation on the
+// next process() pushes the curves again at the new length.
 void HipBackend::updateResamplingCurve(const float* curve, size_t length) {
     if (!curve && length > 0) throw std::runtime_error("Invalid resampling curve pointer");
     requireContext("updateResamplingCurve");
+    if (length != static_cast<size_t>(config_.dataParams.signalLength)) return;
     check(ope_hip_set_resampling_curve(ctx_, curve, length), "updateResamplingCurve");
 }
 
@@ -194,6 +199,7 @@ void HipBackend::updateDispersionCurve(const float* curve, size_t length) {
 void HipBackend::updateWindowCurve(const float* curve, size_t length) {
     if (!curve && length > 0) throw std::runtime_error("Invalid window curve pointer");
     requireContext("updateWindowCurve");
+    if (length != static_cast<size_t>(config_.dataParams.signalLength)) return;
     check(ope_hip_set_window_curve(ctx_, curve, length), "updateWindowCurve");
 }
 

@@ -22,6 +22,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace ope_hip {
 
 // ------------------------------------------------------------------ complex
@@ -150,14 +152,16 @@ typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
 // Loads the 8 consecutive raw samples [8t, 8t+8) of A-scan a (streaming,
-// non-temporal: every input byte is read exactly once).
-__device__ __forceinline__ void load_raw(const void* in, int in_type, int n, long long a, int t,
-                                         uint32_t (&raw)[8]) {
-    if (in_type == IN_U16) {
+// non-temporal: every input byte is read exactly once).  IW is the sample
+// width in bytes; it is a compile-time constant so that the loaded words
+// stay in VGPRs and their first use is the only wait on the load.
+template <int IW>
+__device__ __forceinline__ void load_raw(const void* in, int n, long long a, int t, uint32_t (&raw)[8]) {
+    if constexpr (IW == 2) {
         const u32x4* p = reinterpret_cast<const u32x4*>(static_cast<const uint16_t*>(in) + a * n) + t;
         u32x4 q = __builtin_nontemporal_load(p);
         raw[0] = q.x; raw[1] = q.y; raw[2] = q.z; raw[3] = q.w;
-    } else if (in_type == IN_U8) {
+    } else if constexpr (IW == 1) {
         const u32x2* p = reinterpret_cast<const u32x2*>(static_cast<const uint8_t*>(in) + a * n) + t;
         u32x2 q = __builtin_nontemporal_load(p);
         raw[0] = q.x; raw[1] = q.y;
@@ -170,22 +174,22 @@ __device__ __forceinline__ void load_raw(const void* in, int in_type, int n, lon
     }
 }
 
-__device__ __forceinline__ void convert_raw(int in_type, const uint32_t (&raw)[8], float (&x)[8]) {
-    if (in_type == IN_U16) {
+// Samples to float (cpu_kernels.tpp:33-52): unsigned integers by width;
+// 4-byte words are IEEE floats when `ieee` is set (FLOAT32 input).
+template <int IW>
+__device__ __forceinline__ void convert_raw(bool ieee, const uint32_t (&raw)[8], float (&x)[8]) {
+    if constexpr (IW == 2) {
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             x[2 * i] = (float)(raw[i] & 0xffffu);
             x[2 * i + 1] = (float)(raw[i] >> 16);
         }
-    } else if (in_type == IN_U8) {
+    } else if constexpr (IW == 1) {
 #pragma unroll
         for (int i = 0; i < 8; ++i) x[i] = (float)((raw[i >> 2] >> (8 * (i & 3))) & 0xffu);
-    } else if (in_type == IN_U32) {
-#pragma unroll
-        for (int i = 0; i < 8; ++i) x[i] = (float)raw[i];
     } else {
 #pragma unroll
-        for (int i = 0; i < 8; ++i) x[i] = __uint_as_float(raw[i]);
+        for (int i = 0; i < 8; ++i) x[i] = ieee ? __uint_as_float(raw[i]) : (float)raw[i];
     }
 }
 
@@ -294,11 +298,13 @@ __device__ __forceinline__ void dc_remove(float (&xv)[8], int t, int window, S* 
 #endif
 
 // T = taps per output sample: 1 (no resampling), 2 (linear), 4 (cubic),
-// 16 (Lanczos-8).  MODE: M_* bits.  AS: A-scans each lane carries through
+// 16 (Lanczos-8).  MODE: M_* bits.  IW: input sample width in bytes (1, 2,
+// 4); DC removal sums 1- and 2-byte samples in int, 4-byte ones in double.
+// AS: A-scans each lane carries through
 // the chain at once (instruction-level parallelism: the LDS traffic of one
 // overlaps the arithmetic of the other, and each barrier covers AS A-scans;
 // the per-position tables are shared).
-template <int N, int T, int MODE, int AS>
+template <int N, int T, int MODE, int IW, int AS = 1>
 __global__ void __launch_bounds__(Geo<N>::WG) __attribute__((amdgpu_waves_per_eu(OPE_WAVES_PER_EU)))
 chain_kernel(ChainArgs A) {
     using G = Geo<N>;
@@ -314,7 +320,8 @@ chain_kernel(ChainArgs A) {
     __shared__ float2 bufA[SLOTS * G::PADN];
     __shared__ float2 bufB[SLOTS * G::PADN];
     __shared__ float xsep[X_ALIAS_B ? 1 : SLOTS * N];
-    __shared__ double scan_part[DC ? AS * NW : 1];
+    using DcSum = typename std::conditional<(IW <= 2), int, double>::type;
+    __shared__ DcSum scan_part[DC ? AS * NW : 1];
     __shared__ float mm_part[STATS ? 1 : 2 * AS * NW];
 
     const int slot = threadIdx.x / TPA;
@@ -330,7 +337,7 @@ chain_kernel(ChainArgs A) {
     }
 
     const int flags = A.flags;
-    const int in_type = A.in_type;
+    const bool ieee = A.in_type == IN_F32;
 
     // ---- per-lane tables, read once per launch
     int tb[8];
@@ -416,7 +423,7 @@ chain_kernel(ChainArgs A) {
 #pragma unroll
         for (int i = 0; i < 8; ++i) raw[u][i] = 0;
         const long long a = a_begin + slot * AS + u;
-        if (a < a_end) load_raw(A.in, in_type, N, a, t, raw[u]);
+        if (a < a_end) load_raw<IW>(A.in, N, a, t, raw[u]);
     }
 
     for (long long a0 = a_begin; a0 < a_end; a0 += a_step) {
@@ -424,7 +431,7 @@ chain_kernel(ChainArgs A) {
         float xv[AS][8];
 #pragma unroll
         for (int u = 0; u < AS; ++u) {
-            convert_raw(in_type, raw[u], xv[u]);
+            convert_raw<IW>(ieee, raw[u], xv[u]);
             if constexpr (!DC) {
                 float4* dst = reinterpret_cast<float4*>(xs[u] + 8 * t);
                 dst[0] = make_float4(xv[u][0], xv[u][1], xv[u][2], xv[u][3]);
@@ -432,7 +439,7 @@ chain_kernel(ChainArgs A) {
             }
             // prefetch the next A-scan while this one is processed
             const long long an = a0 + a_step + slot * AS + u;
-            if (an < a_end) load_raw(A.in, in_type, N, an, t, raw[u]);
+            if (an < a_end) load_raw<IW>(A.in, N, an, t, raw[u]);
         }
 
         if constexpr (DC) {
@@ -441,12 +448,8 @@ chain_kernel(ChainArgs A) {
             __syncthreads();
 #pragma unroll
             for (int u = 0; u < AS; ++u) {
-                if (in_type == IN_U8 || in_type == IN_U16)
-                    dc_remove<N, TPA, G::WG, int>(xv[u], t, A.dc_window, reinterpret_cast<int*>(pa[u]),
-                                                  reinterpret_cast<int*>(scan_part + u * NW));
-                else
-                    dc_remove<N, TPA, G::WG, double>(xv[u], t, A.dc_window, reinterpret_cast<double*>(pa[u]),
-                                                     scan_part + u * NW);
+                dc_remove<N, TPA, G::WG, DcSum>(xv[u], t, A.dc_window, reinterpret_cast<DcSum*>(pa[u]),
+                                                scan_part + u * NW);
                 float4* dst = reinterpret_cast<float4*>(xs[u] + 8 * t);
                 dst[0] = make_float4(xv[u][0], xv[u][1], xv[u][2], xv[u][3]);
                 dst[1] = make_float4(xv[u][4], xv[u][5], xv[u][6], xv[u][7]);
@@ -618,78 +621,6 @@ chain_kernel(ChainArgs A) {
             }
         }
     }
-}
-
-// ------------------------------------------------------------------ FPN select
-
-// Minimum-variance segment mean per bin, the float/double mixture of
-// getMinimumVarianceMean (cpu_kernels.tpp:402-458).  acc is [segs][width][3].
-__global__ void fpn_select_kernel(const double* __restrict__ acc, int width, int segments,
-                                  int seg_w, float2* __restrict__ out) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= width) return;
-    const float factor = 1.0f / (float)seg_w;
-    float min_var = INFINITY, bx = 0.f, by = 0.f;
-    for (int s = 0; s < segments; ++s) {
-        const double* d = acc + ((size_t)s * width + k) * 3;
-        const float mx = (float)(d[0] * (double)factor);
-        const float my = (float)(d[1] * (double)factor);
-        const float var = (float)(d[2] * (double)factor - ((double)mx * (double)mx + (double)my * (double)my));
-        if (var < min_var) {
-            min_var = var;
-            bx = mx;
-            by = my;
-        }
-    }
-    out[k] = make_float2(bx, by);
-}
-
-// Segment sums over arbitrary interleaved complex rows [height][width]
-// (for the individual getMinimumVarianceMean operation).
-__global__ void segment_sums_kernel(const float2* __restrict__ in, int width, int seg_w,
-                                    int segments, double* __restrict__ acc) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const int s = blockIdx.y;
-    if (k >= width || s >= segments) return;
-    double sx = 0.0, sy = 0.0, sxx = 0.0;
-    for (int r = 0; r < seg_w; ++r) {
-        const float2 c = in[(size_t)(s * seg_w + r) * width + k];
-        const double dx = c.x, dy = c.y;
-        sx += dx;
-        sy += dy;
-        sxx += dx * dx + dy * dy;
-    }
-    double* d = acc + ((size_t)s * width + k) * 3;
-    d[0] = sx;
-    d[1] = sy;
-    d[2] = sxx;
-}
-
-// ------------------------------------------------------------------ small kernels
-
-__global__ void fill_zero_f64(double* __restrict__ p, long long n) {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0.0;
-}
-
-// Column mean of the processed frame (cpu_backend.cpp:105-118), accumulated
-// in double.
-__global__ void bg_record_kernel(const float* __restrict__ out, long long rows, int width,
-                                 float* __restrict__ bg) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= width) return;
-    double s = 0.0;
-    for (long long r = 0; r < rows; ++r) s += out[r * width + k];
-    bg[k] = (float)(s / (double)rows);
-}
-
-// out -= weight*bg + offset (cpu_kernels.tpp:380-400)
-__global__ void bg_subtract_kernel(float* __restrict__ out, long long total, int width,
-                                   const float* __restrict__ bg, float w, float off) {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= total) return;
-    const float b = __fadd_rn(__fmul_rn(w, bg[i % width]), off);
-    out[i] = out[i] - b;
 }
 
 }  // namespace ope_hip

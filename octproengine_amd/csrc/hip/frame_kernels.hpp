@@ -1,0 +1,83 @@
+// Per-frame helper kernels around the chain: FPN segment selection, the
+// segment sums of the individual minimum-variance operation, accumulator
+// clearing and the post-process background.  Included by the C ABI host
+// translation unit only.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace ope_hip {
+
+// ------------------------------------------------------------------ FPN select
+
+// Minimum-variance segment mean per bin, the float/double mixture of
+// getMinimumVarianceMean (cpu_kernels.tpp:402-458).  acc is [segs][width][3].
+__global__ void fpn_select_kernel(const double* __restrict__ acc, int width, int segments,
+                                  int seg_w, float2* __restrict__ out) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= width) return;
+    const float factor = 1.0f / (float)seg_w;
+    float min_var = INFINITY, bx = 0.f, by = 0.f;
+    for (int s = 0; s < segments; ++s) {
+        const double* d = acc + ((size_t)s * width + k) * 3;
+        const float mx = (float)(d[0] * (double)factor);
+        const float my = (float)(d[1] * (double)factor);
+        const float var = (float)(d[2] * (double)factor - ((double)mx * (double)mx + (double)my * (double)my));
+        if (var < min_var) {
+            min_var = var;
+            bx = mx;
+            by = my;
+        }
+    }
+    out[k] = make_float2(bx, by);
+}
+
+// Segment sums over arbitrary interleaved complex rows [height][width]
+// (for the individual getMinimumVarianceMean operation).
+__global__ void segment_sums_kernel(const float2* __restrict__ in, int width, int seg_w,
+                                    int segments, double* __restrict__ acc) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int s = blockIdx.y;
+    if (k >= width || s >= segments) return;
+    double sx = 0.0, sy = 0.0, sxx = 0.0;
+    for (int r = 0; r < seg_w; ++r) {
+        const float2 c = in[(size_t)(s * seg_w + r) * width + k];
+        const double dx = c.x, dy = c.y;
+        sx += dx;
+        sy += dy;
+        sxx += dx * dx + dy * dy;
+    }
+    double* d = acc + ((size_t)s * width + k) * 3;
+    d[0] = sx;
+    d[1] = sy;
+    d[2] = sxx;
+}
+
+// ------------------------------------------------------------------ small kernels
+
+__global__ void fill_zero_f64(double* __restrict__ p, long long n) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0;
+}
+
+// Column mean of the processed frame (cpu_backend.cpp:105-118), accumulated
+// in double.
+__global__ void bg_record_kernel(const float* __restrict__ out, long long rows, int width,
+                                 float* __restrict__ bg) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= width) return;
+    double s = 0.0;
+    for (long long r = 0; r < rows; ++r) s += out[r * width + k];
+    bg[k] = (float)(s / (double)rows);
+}
+
+// out -= weight*bg + offset (cpu_kernels.tpp:380-400)
+__global__ void bg_subtract_kernel(float* __restrict__ out, long long total, int width,
+                                   const float* __restrict__ bg, float w, float off) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const float b = __fadd_rn(__fmul_rn(w, bg[i % width]), off);
+    out[i] = out[i] - b;
+}
+
+}  // namespace ope_hip

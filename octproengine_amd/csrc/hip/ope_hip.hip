@@ -17,10 +17,12 @@
 #include <mutex>
 #include <new>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../../include/ope_hip.h"
-#include "chain_kernels.hpp"
+#include "chain_dispatch.h"
+#include "frame_kernels.hpp"
 
 using namespace ope_hip;
 
@@ -83,6 +85,10 @@ struct ope_hip_ctx {
     int next_slot = 0;
     std::vector<HostCall> calls;  // per slot
     std::mutex submit_mu;
+
+    // resident workgroups per CU of each chain kernel used so far (the
+    // occupancy query is a runtime call; frames reuse one or two kernels)
+    std::vector<std::pair<const void*, int>> occupancy;
 };
 
 namespace {
@@ -268,43 +274,13 @@ int validate_params(ope_hip_ctx* c, const ope_hip_params* p) {
 
 // ---------------------------------------------------------------- launches
 
-using KernelFn = void (*)(ChainArgs);
-
-// A-scans per lane.  Two (instruction-level parallelism) measured slower
-// than one on gfx950 for every mode: the extra VGPRs and LDS cost a wave of
-// occupancy per SIMD, which hides more latency than the second A-scan does.
-constexpr int kAscansPerLane = 1;
-
-template <int N, int MODE>
-KernelFn pick_t(int taps) {
-    constexpr int AS = kAscansPerLane;
-    switch (taps) {
-        case 1: return chain_kernel<N, 1, MODE, AS>;
-        case 2: return chain_kernel<N, 2, MODE, AS>;
-        case 4: return chain_kernel<N, 4, MODE, AS>;
-        default: return chain_kernel<N, 16, MODE, AS>;
-    }
-}
-
-int ascans_per_lane(int) { return kAscansPerLane; }
-
-template <int N>
-KernelFn pick_mode(int taps, int mode) {
-    switch (mode) {
-        case 0: return pick_t<N, 0>(taps);
-        case M_DC: return pick_t<N, M_DC>(taps);
-        case M_STATS: return pick_t<N, M_STATS>(taps);
-        default: return pick_t<N, M_DC | M_STATS>(taps);
-    }
-}
-
-KernelFn pick_kernel(int n, int taps, int mode, int* wg, int* apw) {
+ChainLaunch pick_kernel(int n, int taps, int mode, int iw) {
     switch (n) {
-        case 256: *wg = Geo<256>::WG; *apw = Geo<256>::APW; return pick_mode<256>(taps, mode);
-        case 512: *wg = Geo<512>::WG; *apw = Geo<512>::APW; return pick_mode<512>(taps, mode);
-        case 1024: *wg = Geo<1024>::WG; *apw = Geo<1024>::APW; return pick_mode<1024>(taps, mode);
-        case 2048: *wg = Geo<2048>::WG; *apw = Geo<2048>::APW; return pick_mode<2048>(taps, mode);
-        default: *wg = Geo<4096>::WG; *apw = Geo<4096>::APW; return pick_mode<4096>(taps, mode);
+        case 256: return pick_chain_n<256>(taps, mode, iw);
+        case 512: return pick_chain_n<512>(taps, mode, iw);
+        case 1024: return pick_chain_n<1024>(taps, mode, iw);
+        case 2048: return pick_chain_n<2048>(taps, mode, iw);
+        default: return pick_chain_n<4096>(taps, mode, iw);
     }
 }
 
@@ -351,17 +327,21 @@ ChainArgs base_args(ope_hip_ctx* c) {
 }
 
 int launch_chain(ope_hip_ctx* c, const ChainArgs& a, bool stats, long long work_ascans, hipStream_t s) {
-    int wg = 0, apw = 1;
     const int mode = (c->p.dc_removal_enabled ? M_DC : 0) | (stats ? M_STATS : 0);
-    KernelFn k = pick_kernel(c->n, c->taps, mode, &wg, &apw);
+    const ChainLaunch l = pick_kernel(c->n, c->taps, mode, (int)input_bytes(c->p.input_type));
+    const void* key = reinterpret_cast<const void*>(l.fn);
     int per_cu = 0;
-    HIP_TRY(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(k), wg, 0));
-    if (per_cu < 1) per_cu = 1;
-    const int slots = apw * ascans_per_lane(mode);
-    long long groups = (work_ascans + slots - 1) / slots;
+    for (const auto& e : c->occupancy)
+        if (e.first == key) per_cu = e.second;
+    if (per_cu == 0) {
+        HIP_TRY(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, key, l.wg, 0));
+        if (per_cu < 1) per_cu = 1;
+        c->occupancy.emplace_back(key, per_cu);
+    }
+    long long groups = (work_ascans + l.slots - 1) / l.slots;
     long long grid = std::min<long long>(groups, (long long)per_cu * c->cu_count);
     if (grid < 1) return OPE_HIP_OK;
-    hipLaunchKernelGGL(k, dim3((unsigned)grid), dim3(wg), 0, s, a);
+    hipLaunchKernelGGL(l.fn, dim3((unsigned)grid), dim3(l.wg), 0, s, a);
     HIP_TRY(c, hipGetLastError());
     return OPE_HIP_OK;
 }

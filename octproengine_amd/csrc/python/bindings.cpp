@@ -13,6 +13,7 @@
 #include <array>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <stdexcept>
 #include <string>
@@ -108,10 +109,17 @@ public:
     }
 
     ope::Processor::CallbackId add_output_callback(py::function cb) {
-        auto wrapped = [cb](const ope::IOBuffer& out) {
+        // The C++ side copies the std::function freely (without the GIL), so
+        // the Python callable is shared through a pointer whose last owner
+        // releases it with the GIL held.
+        std::shared_ptr<py::function> fn(new py::function(cb), [](py::function* f) {
+            py::gil_scoped_acquire gil;
+            delete f;
+        });
+        auto wrapped = [fn](const ope::IOBuffer& out) {
             py::gil_scoped_acquire gil;
             try {
-                cb(buffer_view(const_cast<ope::IOBuffer&>(out)));
+                (*fn)(buffer_view(const_cast<ope::IOBuffer&>(out)));
             } catch (const py::error_already_set& e) {
                 py::print("Python error in callback:", e.what());
             } catch (const std::exception& e) {
