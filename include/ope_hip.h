@@ -197,6 +197,94 @@ OPE_HIP_API int ope_hip_host_free(void* ptr);
 OPE_HIP_API int ope_hip_op_min_variance_mean(ope_hip_ctx* ctx, const float* input, int width,
                                              int height, int segments, float* out);
 
+/* ---- Graph-captured frames.  No counterpart in the reference (one
+ * process() call per frame, cuda_backend.cu:442-773).  For callers that
+ * replay a fixed ring of device frames (volume acquisition into
+ * preallocated buffers): n_frames launches of the chain on d_in[f] ->
+ * d_out[f] captured into one HIP graph, replayed by one call.
+ * Only valid in steady state (no FPN determination, FPN accumulation or
+ * background recording pending: OPE_HIP_ESTATE otherwise); the graph keeps
+ * the context's tables, so curve or parameter changes after capture need a
+ * new graph.  Errors of create go to ope_hip_graph_last_error(NULL). */
+typedef struct ope_hip_graph ope_hip_graph;
+OPE_HIP_API int ope_hip_graph_create(ope_hip_ctx* ctx, const void* const* d_in, float* const* d_out,
+                                     int n_frames, int64_t ascans_per_frame, ope_hip_graph** out);
+OPE_HIP_API int ope_hip_graph_launch(ope_hip_graph* graph, void* stream);
+OPE_HIP_API const char* ope_hip_graph_last_error(const ope_hip_graph* graph);
+OPE_HIP_API int ope_hip_graph_destroy(ope_hip_graph* graph);
+
+/* ---- Ragged batches.  No counterpart in the reference, whose backends
+ * process one fixed-size frame per call (ProcessingBackend::process,
+ * backend_interface.h:27).  A ragged batch runs many requests of different
+ * signal lengths and A-scan counts in one persistent launch: for serving
+ * callers that queue requests from several scan protocols. */
+
+/* One request: n_ascans A-scans of signal_length samples from d_in (device,
+ * [n_ascans][signal_length] in the contexts' input type, 16-byte aligned)
+ * to d_out (device, [n_ascans][signal_length/2] float, 4-byte aligned). */
+typedef struct ope_hip_request {
+    const void* d_in;
+    float* d_out;
+    int64_t n_ascans;
+    int32_t signal_length;
+    int32_t reserved; /* 0 */
+} ope_hip_request;
+
+typedef struct ope_hip_ragged ope_hip_ragged;
+
+/* Builds a batch over `reqs` (copied).  ctxs[0..n_ctx) supply the chain of
+ * each signal length (tables, epilogue, FPN profile and background as they
+ * are at this call); their lengths must differ, and they must agree on the
+ * interpolation method, DC removal and input sample width, and live on one
+ * device.  A context with an FPN determination or background recording
+ * pending is rejected (OPE_HIP_ESTATE): run it through process_device
+ * first.  chunk_ascans: A-scans per task (<= 0: 2^18 samples per task).
+ * Errors are reported through ope_hip_last_error(NULL). */
+OPE_HIP_API int ope_hip_ragged_create(ope_hip_ctx* const* ctxs, int n_ctx, const ope_hip_request* reqs,
+                                      int n_reqs, int chunk_ascans, ope_hip_ragged** out);
+
+/* One launch over the whole batch on `stream` (NULL: the HIP null stream).
+ * Asynchronous, allocates nothing and may be captured in a graph.  Runs of
+ * one batch must not overlap (they share its task counters). */
+OPE_HIP_API int ope_hip_ragged_run(ope_hip_ragged* batch, void* stream);
+
+/* Per-request latency of the last completed run: microseconds from the
+ * moment the first task was taken to the completion of the request's last
+ * A-scan (0 for empty requests).  Call after the run has finished.
+ * latency_us has n_reqs entries. */
+OPE_HIP_API int ope_hip_ragged_latencies(ope_hip_ragged* batch, double* latency_us);
+
+OPE_HIP_API const char* ope_hip_ragged_last_error(const ope_hip_ragged* batch);
+OPE_HIP_API int ope_hip_ragged_destroy(ope_hip_ragged* batch);
+
+/* ---- Autofocus.  No counterpart in the reference, which takes dispersion
+ * and resampling coefficients as given (setDispersionCoefficients,
+ * setResamplingCoefficients, processor.h).  A differentiable form of the
+ * chain for callers that tune those coefficients by gradient descent on
+ * image sharpness.
+ *
+ * theta[6] = (d2, d3, c0, c1, c2, c3):
+ *   resampling curve r[j] = clamp(c0 + c1 u + c2 u^2 + c3 u^3, 7, N-9),
+ *   u = j/(N-1) (the polynomial of generateResamplingCurve,
+ *   processorconfiguration.cpp:438-459); dispersion phase
+ *   phi[j] = d2 (j-N/2)^2 + d3 (j-N/2)^3.
+ * The chain is the context's: input type (16- or 32-bit), DC removal,
+ * window, and linear or cubic interpolation (resampling must be enabled);
+ * the context's curves and FPN/background state are not used.  The loss is
+ * the Shannon entropy of the intensity |X[k]|^2, k < N/2, normalised over
+ * all n_ascans A-scans of d_in (device, 16-byte aligned).
+ * Both calls run on `stream` and return after the result is on the host. */
+
+/* out[3] = (loss, S = sum of intensities, sum of I log I). */
+OPE_HIP_API int ope_hip_autofocus_loss(ope_hip_ctx* ctx, const void* d_in, int64_t n_ascans,
+                                       const double* theta, double* out, void* stream);
+
+/* grad[6] = dloss/dtheta at theta, given S and the loss of the same theta
+ * from ope_hip_autofocus_loss (the hand-written backward of the chain). */
+OPE_HIP_API int ope_hip_autofocus_grad(ope_hip_ctx* ctx, const void* d_in, int64_t n_ascans,
+                                       const double* theta, double sum_i, double loss, double* grad,
+                                       void* stream);
+
 #ifdef __cplusplus
 }
 #endif

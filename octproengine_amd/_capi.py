@@ -50,6 +50,18 @@ SIGNATURES = {
     "ope_hip_op_min_variance_mean": (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_void_p]),
     "ope_hip_host_alloc": (c_int, [c_size_t, POINTER(c_void_p)]),
     "ope_hip_host_free": (c_int, [c_void_p]),
+    "ope_hip_autofocus_loss": (c_int, [c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_void_p]),
+    "ope_hip_autofocus_grad": (c_int, [c_void_p, c_void_p, c_int64, c_void_p, ctypes.c_double,
+                                       ctypes.c_double, c_void_p, c_void_p]),
+    "ope_hip_graph_create": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int64, POINTER(c_void_p)]),
+    "ope_hip_graph_launch": (c_int, [c_void_p, c_void_p]),
+    "ope_hip_graph_last_error": (c_char_p, [c_void_p]),
+    "ope_hip_graph_destroy": (c_int, [c_void_p]),
+    "ope_hip_ragged_create": (c_int, [c_void_p, c_int, c_void_p, c_int, c_int, POINTER(c_void_p)]),
+    "ope_hip_ragged_run": (c_int, [c_void_p, c_void_p]),
+    "ope_hip_ragged_latencies": (c_int, [c_void_p, c_void_p]),
+    "ope_hip_ragged_last_error": (c_char_p, [c_void_p]),
+    "ope_hip_ragged_destroy": (c_int, [c_void_p]),
 }
 
 
@@ -70,6 +82,12 @@ class Params(Structure):
         ("fpn_bscan_count", c_int32), ("fpn_continuous", c_int32), ("bg_removal", c_int32),
         ("bg_weight", c_float), ("bg_offset", c_float),
     ]
+
+
+class Request(Structure):
+    """ope_hip_request (include/ope_hip.h)."""
+    _fields_ = [("d_in", c_void_p), ("d_out", c_void_p), ("n_ascans", c_int64),
+                ("signal_length", c_int32), ("reserved", c_int32)]
 
 
 def make_params(**kw) -> Params:
@@ -229,3 +247,91 @@ class Context:
 
     def synchronize(self) -> None:
         self._check(lib.ope_hip_synchronize(self._h))
+
+    # -- autofocus (theta = d2, d3, c0, c1, c2, c3)
+    def autofocus_loss(self, d_in: int, n_ascans: int, theta, stream: int = 0):
+        """(loss, S, sum I log I) of the entropy of the frame at d_in."""
+        th = np.ascontiguousarray(theta, dtype=np.float64)
+        out = np.zeros(3, dtype=np.float64)
+        self._check(lib.ope_hip_autofocus_loss(self._h, c_void_p(d_in), int(n_ascans), th.ctypes.data,
+                                               out.ctypes.data, c_void_p(stream or None)))
+        return float(out[0]), float(out[1]), float(out[2])
+
+    def autofocus_grad(self, d_in: int, n_ascans: int, theta, sum_i: float, loss: float,
+                       stream: int = 0) -> np.ndarray:
+        th = np.ascontiguousarray(theta, dtype=np.float64)
+        out = np.zeros(6, dtype=np.float64)
+        self._check(lib.ope_hip_autofocus_grad(self._h, c_void_p(d_in), int(n_ascans), th.ctypes.data,
+                                               float(sum_i), float(loss), out.ctypes.data,
+                                               c_void_p(stream or None)))
+        return out
+
+
+class RaggedBatch:
+    """ope_hip_ragged: requests of mixed signal length in one persistent
+    launch.  `contexts` supply the chain of each length; `requests` is a
+    sequence of (d_in, d_out, n_ascans, signal_length) with device pointers
+    as ints."""
+
+    def __init__(self, contexts, requests, chunk_ascans: int = 0):
+        self.contexts = list(contexts)  # kept alive: the batch points at their tables
+        handles = (c_void_p * len(self.contexts))(*[c._h for c in self.contexts])
+        reqs = (Request * max(len(requests), 1))()
+        for i, (d_in, d_out, n, length) in enumerate(requests):
+            reqs[i] = Request(c_void_p(d_in), c_void_p(d_out), int(n), int(length), 0)
+        self.n_reqs = len(requests)
+        h = c_void_p()
+        rc = lib.ope_hip_ragged_create(handles, len(self.contexts), reqs, self.n_reqs, int(chunk_ascans),
+                                       ctypes.byref(h))
+        if rc != OK:
+            raise HipError(rc, lib.ope_hip_ragged_last_error(None).decode())
+        self._h = h
+
+    def _check(self, rc: int) -> None:
+        if rc != OK:
+            raise HipError(rc, lib.ope_hip_ragged_last_error(self._h).decode())
+
+    def run(self, stream: int = 0) -> None:
+        self._check(lib.ope_hip_ragged_run(self._h, c_void_p(stream or None)))
+
+    def latencies_us(self) -> np.ndarray:
+        out = np.zeros(max(self.n_reqs, 1), dtype=np.float64)
+        self._check(lib.ope_hip_ragged_latencies(self._h, out.ctypes.data))
+        return out[:self.n_reqs]
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) and lib is not None:
+            lib.ope_hip_ragged_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        self.close()
+
+
+class FrameGraph:
+    """ope_hip_graph: the chain over a fixed ring of device frames captured
+    into one HIP graph (steady state only)."""
+
+    def __init__(self, context: Context, d_in, d_out, ascans_per_frame: int):
+        self.context = context
+        n = len(d_in)
+        ins = (c_void_p * n)(*[c_void_p(p) for p in d_in])
+        outs = (c_void_p * n)(*[c_void_p(p) for p in d_out])
+        h = c_void_p()
+        rc = lib.ope_hip_graph_create(context._h, ins, outs, n, int(ascans_per_frame), ctypes.byref(h))
+        if rc != OK:
+            raise HipError(rc, lib.ope_hip_graph_last_error(None).decode())
+        self._h = h
+
+    def launch(self, stream: int = 0) -> None:
+        rc = lib.ope_hip_graph_launch(self._h, c_void_p(stream or None))
+        if rc != OK:
+            raise HipError(rc, lib.ope_hip_graph_last_error(self._h).decode())
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) and lib is not None:
+            lib.ope_hip_graph_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        self.close()

@@ -34,6 +34,7 @@ HIP_FLAGS = [
     "-munsafe-fp-atomics", "-Wall", "-Wno-unused-function",
 ]
 CHAIN_LENGTHS = (256, 512, 1024, 2048, 4096)
+RAGGED_TAPS = (1, 2, 4, 16)
 
 LIB_HIP = PKG / "libope_hip.so"
 LIB_CORE = PKG / "libope.so"
@@ -63,8 +64,10 @@ def _run_parallel(cmds: list[list[str]], jobs: int) -> None:
 
 
 def build_hip(verbose_resources: bool = False) -> Path:
-    """libope_hip.so: the C ABI host code plus one object per signal length
-    holding that length's chain-kernel instantiations, compiled in parallel."""
+    """libope_hip.so: the C ABI host code, one object per signal length
+    holding that length's chain-kernel instantiations and one object per tap
+    count holding the ragged-batch kernels, one object per signal length
+    holding the autofocus kernels, compiled in parallel."""
     hip_dir = CSRC / "hip"
     OBJ_DIR.mkdir(exist_ok=True)
     extra = ["-Rpass-analysis=kernel-resource-usage"] if verbose_resources else []
@@ -76,6 +79,16 @@ def build_hip(verbose_resources: bool = False) -> Path:
         obj = OBJ_DIR / f"chain_n{n}.o"
         cmds.append([HIPCC, *HIP_FLAGS, *extra, f"-I{INCLUDE}", f"-DOPE_CHAIN_N={n}", "-c",
                      str(hip_dir / "chain_instances.hip"), "-o", str(obj)])
+        objs.append(obj)
+    for t in RAGGED_TAPS:
+        obj = OBJ_DIR / f"ragged_t{t}.o"
+        cmds.append([HIPCC, *HIP_FLAGS, *extra, f"-I{INCLUDE}", f"-DOPE_RAGGED_T={t}", "-c",
+                     str(hip_dir / "ragged_instances.hip"), "-o", str(obj)])
+        objs.append(obj)
+    for n in CHAIN_LENGTHS:
+        obj = OBJ_DIR / f"autofocus_n{n}.o"
+        cmds.append([HIPCC, *HIP_FLAGS, *extra, f"-I{INCLUDE}", f"-DOPE_AF_N={n}", "-c",
+                     str(hip_dir / "autofocus_instances.hip"), "-o", str(obj)])
         objs.append(obj)
     jobs = max(1, min(len(cmds), int(os.environ.get("MAX_JOBS", os.cpu_count() or 1))))
     _run_parallel(cmds, jobs)

@@ -4,6 +4,8 @@
 #pragma once
 
 #include "chain_kernels.hpp"
+#include "ragged_kernels.hpp"
+#include "autofocus_kernels.hpp"
 
 namespace ope_hip {
 
@@ -19,5 +21,28 @@ struct ChainLaunch {
 // bytes (1, 2, 4).  Explicitly instantiated for N = 256 ... 4096.
 template <int N>
 ChainLaunch pick_chain_n(int taps, int mode, int iw);
+
+using RaggedFn = void (*)(RaggedArgs);
+
+// Ragged-batch kernel for taps T, mode (0 or M_DC) and input width iw.
+// Explicitly instantiated for T = 1, 2, 4, 16 (ragged_instances.hip
+// compiled with -DOPE_RAGGED_T=T).
+template <int T>
+RaggedFn pick_ragged_t(int mode, int iw);
+
+using AutofocusFn = void (*)(AutofocusArgs);
+
+struct AutofocusLaunch {
+    AutofocusFn fn;
+    int wg;
+    int slots;
+};
+
+// Autofocus forward (bwd = false) or backward kernel of length N for taps 2
+// (linear) or 4 (cubic), input width iw (2 or 4 bytes) and DC removal on or
+// off.  Explicitly instantiated for N = 256 ... 4096 (autofocus_instances.hip
+// compiled with -DOPE_AF_N=N).
+template <int N>
+AutofocusLaunch pick_autofocus_n(int taps, int iw, bool dc, bool bwd);
 
 }  // namespace ope_hip

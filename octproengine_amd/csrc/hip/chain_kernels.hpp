@@ -95,8 +95,8 @@ template <int N>
 struct Geo {
     static constexpr int LOG2N = ilog2(N);
     static constexpr int TPA = N / 8;                       // lanes per A-scan
-    static constexpr int WG = TPA >= 256 ? TPA : 256;       // workgroup size
-    static constexpr int APW = WG / TPA;                    // A-scans per workgroup
+    static constexpr int WG = TPA >= 256 ? TPA : 256;       // workgroup size of chain_kernel
+    static constexpr int APW = WG / TPA;                    // A-scans per chain_kernel workgroup
     static constexpr int NPASS = (LOG2N + 2) / 3;           // Stockham passes
     static constexpr int LAST_R = N >> (3 * (NPASS - 1));   // radix of the last pass
     // padded float2 slots; at least N+1 so buffer A also holds the N+1
@@ -146,6 +146,30 @@ struct ChainArgs {
 // input classes (numeric values match ope::DataType)
 enum : int { IN_U8 = 1, IN_U16 = 2, IN_U32 = 3, IN_F32 = 9 };
 
+// ------------------------------------------------------------------ global memory
+
+// Pointers that live in argument structs are generic; every HBM access of
+// the chain goes through these casts so that it is a global (not flat)
+// instruction also where the body is not inlined into a kernel.
+#define OPE_GLOBAL __attribute__((address_space(1)))
+
+template <typename T>
+__device__ __forceinline__ const OPE_GLOBAL T* gmem(const T* p) {
+    return (const OPE_GLOBAL T*)p;
+}
+template <typename T>
+__device__ __forceinline__ OPE_GLOBAL T* gmem(T* p) {
+    return (OPE_GLOBAL T*)p;
+}
+
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+
+// p[i] of a float2 table through a global load
+__device__ __forceinline__ float2 gload2(const float2* p, int i) {
+    const f32x2 v = *gmem(reinterpret_cast<const f32x2*>(p) + i);
+    return make_float2(v.x, v.y);
+}
+
 // ------------------------------------------------------------------ loads
 
 typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
@@ -158,15 +182,15 @@ typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 template <int IW>
 __device__ __forceinline__ void load_raw(const void* in, int n, long long a, int t, uint32_t (&raw)[8]) {
     if constexpr (IW == 2) {
-        const u32x4* p = reinterpret_cast<const u32x4*>(static_cast<const uint16_t*>(in) + a * n) + t;
+        const OPE_GLOBAL u32x4* p = gmem(reinterpret_cast<const u32x4*>(static_cast<const uint16_t*>(in) + a * n) + t);
         u32x4 q = __builtin_nontemporal_load(p);
         raw[0] = q.x; raw[1] = q.y; raw[2] = q.z; raw[3] = q.w;
     } else if constexpr (IW == 1) {
-        const u32x2* p = reinterpret_cast<const u32x2*>(static_cast<const uint8_t*>(in) + a * n) + t;
+        const OPE_GLOBAL u32x2* p = gmem(reinterpret_cast<const u32x2*>(static_cast<const uint8_t*>(in) + a * n) + t);
         u32x2 q = __builtin_nontemporal_load(p);
         raw[0] = q.x; raw[1] = q.y;
     } else {
-        const u32x4* p = reinterpret_cast<const u32x4*>(static_cast<const uint32_t*>(in) + a * n) + 2 * t;
+        const OPE_GLOBAL u32x4* p = gmem(reinterpret_cast<const u32x4*>(static_cast<const uint32_t*>(in) + a * n) + 2 * t);
         u32x4 q0 = __builtin_nontemporal_load(p);
         u32x4 q1 = __builtin_nontemporal_load(p + 1);
         raw[0] = q0.x; raw[1] = q0.y; raw[2] = q0.z; raw[3] = q0.w;
@@ -197,7 +221,7 @@ __device__ __forceinline__ void convert_raw(bool ieee, const uint32_t (&raw)[8],
 
 // Inclusive scan of v over the TPA lanes of one A-scan.  `part` is LDS
 // scratch of WG/64 entries (one per wavefront).  Called by all lanes.
-template <typename T, int TPA, int WG>
+template <typename T, int TPA>
 __device__ __forceinline__ T ascan_inclusive_scan(T v, T* part) {
     const int lane = threadIdx.x & 63;
     constexpr int W = TPA < 64 ? TPA : 64;
@@ -220,7 +244,7 @@ __device__ __forceinline__ T ascan_inclusive_scan(T v, T* part) {
 }
 
 // min/max reduction over the TPA lanes of one A-scan (result in all lanes).
-template <int TPA, int WG>
+template <int TPA>
 __device__ __forceinline__ void ascan_minmax(float& mn, float& mx, float* part_mn, float* part_mx) {
     constexpr int W = TPA < 64 ? TPA : 64;
 #pragma unroll
@@ -248,6 +272,107 @@ __device__ __forceinline__ void ascan_minmax(float& mn, float& mx, float* part_m
 
 // ------------------------------------------------------------------ kernel
 
+// ------------------------------------------------------------------ FFT
+
+// Twiddles of Stockham passes 1..NPASS-1 for lane t (pass 0 has Ns = 1: all
+// ones); elements with r == 0 need none.  table = exp(+2 pi i k / N).
+template <int N>
+__device__ __forceinline__ void load_twiddles(const float2* table, int t, cf (&tw)[Geo<N>::NPASS][8]) {
+    using G = Geo<N>;
+    constexpr int NP = G::NPASS;
+    constexpr int TPA = G::TPA;
+#pragma unroll
+    for (int p = 1; p < NP; ++p) {
+        const int R = (p < NP - 1) ? 8 : G::LAST_R;
+        const int RQ = 8 / R;
+        const int ns = 1 << (3 * p);
+        const int step = N / (ns * R);
+#pragma unroll
+        for (int e = RQ; e < 8; ++e) {
+            const int q = e % RQ, r = e / RQ;
+            const int b = t + q * TPA;
+            const int ex = (r * (b & (ns - 1)) * step) & (N - 1);
+            float2 w = gload2(table, ex);
+            tw[p][e] = {w.x, w.y};
+        }
+    }
+}
+
+// Unnormalised inverse FFT (sign +) of the AS sequences whose elements
+// t + e*N/8 lane t holds in v[u][e]; on return v[u][e] = X[t + e*N/8].
+// Radix-8 Stockham passes (the last one of radix 8, 4 or 2) through the
+// padded LDS buffers pa/pb (pass p writes pa for even p, pb for odd p).
+// Ends without a barrier: the last pass's reads of pa/pb may still be in
+// flight in other wavefronts.
+template <int N, int AS>
+__device__ __forceinline__ void stockham_inverse(cf (&v)[AS][8], float2* const (&pa)[AS], float2* const (&pb)[AS],
+                                                 const cf (&tw)[Geo<N>::NPASS][8], int rbase) {
+    using G = Geo<N>;
+    constexpr int NP = G::NPASS;
+    constexpr int TPA = G::TPA;
+    const int t = threadIdx.x % TPA;
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+        const int R = (p < NP - 1) ? 8 : G::LAST_R;
+        const int RQ = 8 / R;
+        const int ns = 1 << (3 * p);
+        if (p > 0) {
+#pragma unroll
+            for (int u = 0; u < AS; ++u) {
+                // pass p>0 reads what pass p-1 wrote
+                const float2* rp = ((p % 2 == 1) ? pa[u] : pb[u]) + rbase;
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    // t < TPA and e*TPA share no bits, so the padded
+                    // index splits into a lane base plus a constant
+                    float2 w = rp[pad_slots(e * TPA)];
+                    v[u][e] = {w.x, w.y};
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < AS; ++u) {
+#pragma unroll
+                for (int e = RQ; e < 8; ++e) v[u][e] = cmul(v[u][e], tw[p][e]);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < AS; ++u) {
+            if (R == 8) {
+                idft8(v[u]);
+            } else if (R == 4) {
+                idft4(v[u][0], v[u][2], v[u][4], v[u][6]);
+                idft4(v[u][1], v[u][3], v[u][5], v[u][7]);
+            } else {
+                idft2(v[u][0], v[u][4]);
+                idft2(v[u][1], v[u][5]);
+                idft2(v[u][2], v[u][6]);
+                idft2(v[u][3], v[u][7]);
+            }
+        }
+        if (p < NP - 1) {
+            // Stockham placement of output m of butterfly b:
+            //   (b / ns) * ns * R + b % ns + m * ns, padded
+#pragma unroll
+            for (int u = 0; u < AS; ++u) {
+                float2* dst = (p % 2 == 0) ? pa[u] : pb[u];
+#pragma unroll
+                for (int q = 0; q < RQ; ++q) {
+                    const int b = t + q * TPA;
+                    float2* wp = dst + pad_idx((b / ns) * ns * R + (b & (ns - 1)));
+#pragma unroll
+                    for (int m = 0; m < R; ++m) {
+                        const int e = q + RQ * m;
+                        // base has no bits where m*ns has any: the padded
+                        // index splits into base and a constant
+                        wp[pad_slots(m * ns)] = make_float2(v[u][e].x, v[u][e].y);
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
 // Compile-time modes of the chain kernel.
 enum : int {
     M_DC = 1,      // rolling-average DC removal prologue
@@ -257,7 +382,7 @@ enum : int {
 // Rolling-average DC removal of the lane's 8 samples (cpu_kernels.tpp:56-82).
 // Window sums come from exact prefix sums (int for 8/16-bit input, double
 // otherwise) built in LDS scratch `P` (N+1 entries).
-template <int N, int TPA, int WG, typename S>
+template <int N, int TPA, typename S>
 __device__ __forceinline__ void dc_remove(float (&xv)[8], int t, int window, S* P, S* part) {
     S loc[8];
     S s = S(0);
@@ -266,7 +391,7 @@ __device__ __forceinline__ void dc_remove(float (&xv)[8], int t, int window, S* 
         s += (S)xv[i];
         loc[i] = s;
     }
-    const S incl = ascan_inclusive_scan<S, TPA, WG>(s, part);
+    const S incl = ascan_inclusive_scan<S, TPA>(s, part);
     const S excl = incl - s;
     if (t == 0) P[0] = S(0);
 #pragma unroll
@@ -297,32 +422,50 @@ __device__ __forceinline__ void dc_remove(float (&xv)[8], int t, int window, S* 
 #define OPE_WAVES_PER_EU 1
 #endif
 
+// LDS arrays of one chain body with a workgroup of WG lanes, in elements:
+// two padded complex buffers per A-scan slot, the float staging buffer when
+// it cannot alias buffer B, and the per-wavefront partials of the DC scan
+// and of the auto min/max.  The kernels declare them as separate
+// __shared__ arrays: offsets into them then fold into the LDS instructions'
+// immediate fields, which a single byte array carved at run time loses.
+template <int N, int MODE, int IW, int AS, int WG>
+struct ChainLds {
+    using G = Geo<N>;
+    static_assert(WG % G::TPA == 0 && WG % 64 == 0, "workgroup must hold whole A-scans");
+    static constexpr int SLOTS = (WG / G::TPA) * AS;        // A-scans per workgroup iteration
+    static constexpr int NW = WG / 64;                      // wavefronts per workgroup
+    static constexpr bool X_ALIAS_B = (G::NPASS % 2) == 0;  // x staging can share buffer B
+    using DcSum = typename std::conditional<(IW <= 2), int, double>::type;
+    static constexpr int BUF = SLOTS * G::PADN;                          // float2 per buffer
+    static constexpr int X = X_ALIAS_B ? 1 : SLOTS * N;                 // floats
+    static constexpr int SCAN = (MODE & M_DC) ? AS * NW : 1;            // DcSum
+    static constexpr int MM = (MODE & M_STATS) ? 1 : 2 * AS * NW;       // floats
+};
+
+// The chain over A-scans a_begin + k*a_step ... (< a_end) of A, each
+// workgroup iteration taking SLOTS consecutive A-scans.
+//
 // T = taps per output sample: 1 (no resampling), 2 (linear), 4 (cubic),
 // 16 (Lanczos-8).  MODE: M_* bits.  IW: input sample width in bytes (1, 2,
 // 4); DC removal sums 1- and 2-byte samples in int, 4-byte ones in double.
-// AS: A-scans each lane carries through
-// the chain at once (instruction-level parallelism: the LDS traffic of one
-// overlaps the arithmetic of the other, and each barrier covers AS A-scans;
-// the per-position tables are shared).
-template <int N, int T, int MODE, int IW, int AS = 1>
-__global__ void __launch_bounds__(Geo<N>::WG) __attribute__((amdgpu_waves_per_eu(OPE_WAVES_PER_EU)))
-chain_kernel(ChainArgs A) {
+// AS: A-scans each lane carries through the chain at once (instruction-level
+// parallelism: the LDS traffic of one overlaps the arithmetic of the other,
+// and each barrier covers AS A-scans; the per-position tables are shared).
+// WG: workgroup size, a multiple of N/8.  All lanes of the workgroup call
+// this with the same range.
+template <int N, int T, int MODE, int IW, int AS, int WG>
+__device__ __forceinline__ void chain_body(const ChainArgs A, float2* bufA, float2* bufB, float* xsep,
+                                           typename ChainLds<N, MODE, IW, AS, WG>::DcSum* scan_part,
+                                           float* mm_part, long long a_begin, long long a_end, long long a_step) {
     using G = Geo<N>;
+    using L = ChainLds<N, MODE, IW, AS, WG>;
     constexpr int TPA = G::TPA;
-    constexpr int APW = G::APW;
     constexpr int NP = G::NPASS;
-    constexpr int SLOTS = APW * AS;             // A-scans per workgroup iteration
-    constexpr int NW = G::WG / 64;              // wavefronts per workgroup
+    constexpr int NW = L::NW;
     constexpr bool DC = (MODE & M_DC) != 0;
     constexpr bool STATS = (MODE & M_STATS) != 0;
-    constexpr bool X_ALIAS_B = (NP % 2) == 0;  // x staging can share buffer B
-
-    __shared__ float2 bufA[SLOTS * G::PADN];
-    __shared__ float2 bufB[SLOTS * G::PADN];
-    __shared__ float xsep[X_ALIAS_B ? 1 : SLOTS * N];
-    using DcSum = typename std::conditional<(IW <= 2), int, double>::type;
-    __shared__ DcSum scan_part[DC ? AS * NW : 1];
-    __shared__ float mm_part[STATS ? 1 : 2 * AS * NW];
+    constexpr bool X_ALIAS_B = L::X_ALIAS_B;
+    using DcSum = typename L::DcSum;
 
     const int slot = threadIdx.x / TPA;
     const int t = threadIdx.x - slot * TPA;
@@ -339,41 +482,27 @@ chain_kernel(ChainArgs A) {
     const int flags = A.flags;
     const bool ieee = A.in_type == IN_F32;
 
-    // ---- per-lane tables, read once per launch
-    int tb[8];
+    // ---- per-lane tables, read once per call
+    // tap windows as LDS pointers into the lane's first staging buffer (the
+    // other A-scans of the lane sit at constant offsets from it)
+    const float* tb[8];
     float tw8[8][T];
     cf ph[8];
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
         const int j = t + e * TPA;
-        tb[e] = A.tap_base[j];
+        tb[e] = xs[0] + gmem(A.tap_base)[j];
 #pragma unroll
-        for (int i = 0; i < T; ++i) tw8[e][i] = A.tap_w[j * T + i];
+        for (int i = 0; i < T; ++i) tw8[e][i] = gmem(A.tap_w)[j * T + i];
         if (flags & F_DISP) {
-            float2 p = A.phasor[j];
+            float2 p = gload2(A.phasor, j);
             ph[e] = {p.x, p.y};
         } else {
             ph[e] = {1.f, 0.f};
         }
     }
-    // twiddles of passes 1..NP-1 (pass 0 has Ns = 1: all ones); elements
-    // with r == 0 need none.
     cf tw[NP][8];
-#pragma unroll
-    for (int p = 1; p < NP; ++p) {
-        const int R = (p < NP - 1) ? 8 : G::LAST_R;
-        const int RQ = 8 / R;
-        const int ns = 1 << (3 * p);
-        const int step = N / (ns * R);
-#pragma unroll
-        for (int e = RQ; e < 8; ++e) {
-            const int q = e % RQ, r = e / RQ;
-            const int b = t + q * TPA;
-            const int ex = (r * (b & (ns - 1)) * step) & (N - 1);
-            float2 w = A.twiddle[ex];
-            tw[p][e] = {w.x, w.y};
-        }
-    }
+    load_twiddles<N>(A.twiddle, t, tw);
     cf fp[4];
     float bgv[4];
     if constexpr (!STATS) {
@@ -381,33 +510,18 @@ chain_kernel(ChainArgs A) {
         for (int o = 0; o < 4; ++o) {
             const int k = t + o * TPA;
             if (flags & F_FPN) {
-                float2 f = A.fpn[k];
+                float2 f = gload2(A.fpn, k);
                 fp[o] = {f.x, f.y};
             } else {
                 fp[o] = {0.f, 0.f};
             }
-            bgv[o] = (flags & F_BG) ? __fadd_rn(__fmul_rn(A.bg_w, A.bg[k]), A.bg_off) : 0.f;
+            bgv[o] = (flags & F_BG) ? __fadd_rn(__fmul_rn(A.bg_w, gmem(A.bg)[k]), A.bg_off) : 0.f;
         }
     }
 
     // LDS base: every pass reads positions t + e*TPA (base + constant)
     const int rbase = pad_idx(t);
 
-    long long a_begin, a_step, a_end;
-    if constexpr (STATS) {
-        // contiguous chunks per workgroup so segment runs are long
-        const long long n_work = A.stat_rows;
-        const long long groups = (n_work + SLOTS - 1) / SLOTS;
-        const long long per = (groups + gridDim.x - 1) / gridDim.x;
-        a_begin = (long long)blockIdx.x * per * SLOTS;
-        a_end = a_begin + per * SLOTS;
-        if (a_end > n_work) a_end = n_work;
-        a_step = SLOTS;
-    } else {
-        a_begin = (long long)blockIdx.x * SLOTS;
-        a_end = A.n_ascans;
-        a_step = (long long)gridDim.x * SLOTS;
-    }
     double sx[AS][4], sy[AS][4], sxx[AS][4];
     int cur_seg[AS];
 #pragma unroll
@@ -448,7 +562,7 @@ chain_kernel(ChainArgs A) {
             __syncthreads();
 #pragma unroll
             for (int u = 0; u < AS; ++u) {
-                dc_remove<N, TPA, G::WG, DcSum>(xv[u], t, A.dc_window, reinterpret_cast<DcSum*>(pa[u]),
+                dc_remove<N, TPA, DcSum>(xv[u], t, A.dc_window, reinterpret_cast<DcSum*>(pa[u]),
                                                 scan_part + u * NW);
                 float4* dst = reinterpret_cast<float4*>(xs[u] + 8 * t);
                 dst[0] = make_float4(xv[u][0], xv[u][1], xv[u][2], xv[u][3]);
@@ -463,7 +577,7 @@ chain_kernel(ChainArgs A) {
         for (int u = 0; u < AS; ++u) {
 #pragma unroll
             for (int e = 0; e < 8; ++e) {
-                const float* src = xs[u] + tb[e];
+                const float* src = tb[e] + (xs[u] - xs[0]);
                 float acc = 0.f;
 #pragma unroll
                 for (int i = 0; i < T; ++i) acc = fmaf(tw8[e][i], src[i], acc);
@@ -472,66 +586,7 @@ chain_kernel(ChainArgs A) {
         }
 
         // ---- inverse FFT: NP Stockham passes
-#pragma unroll
-        for (int p = 0; p < NP; ++p) {
-            const int R = (p < NP - 1) ? 8 : G::LAST_R;
-            const int RQ = 8 / R;
-            const int ns = 1 << (3 * p);
-            if (p > 0) {
-#pragma unroll
-                for (int u = 0; u < AS; ++u) {
-                    // pass p>0 reads what pass p-1 wrote
-                    const float2* rp = ((p % 2 == 1) ? pa[u] : pb[u]) + rbase;
-#pragma unroll
-                    for (int e = 0; e < 8; ++e) {
-                        // t < TPA and e*TPA share no bits, so the padded
-                        // index splits into a lane base plus a constant
-                        float2 w = rp[pad_slots(e * TPA)];
-                        v[u][e] = {w.x, w.y};
-                    }
-                }
-#pragma unroll
-                for (int u = 0; u < AS; ++u) {
-#pragma unroll
-                    for (int e = RQ; e < 8; ++e) v[u][e] = cmul(v[u][e], tw[p][e]);
-                }
-            }
-#pragma unroll
-            for (int u = 0; u < AS; ++u) {
-                if (R == 8) {
-                    idft8(v[u]);
-                } else if (R == 4) {
-                    idft4(v[u][0], v[u][2], v[u][4], v[u][6]);
-                    idft4(v[u][1], v[u][3], v[u][5], v[u][7]);
-                } else {
-                    idft2(v[u][0], v[u][4]);
-                    idft2(v[u][1], v[u][5]);
-                    idft2(v[u][2], v[u][6]);
-                    idft2(v[u][3], v[u][7]);
-                }
-            }
-            if (p < NP - 1) {
-                // Stockham placement of output m of butterfly b:
-                //   (b / ns) * ns * R + b % ns + m * ns, padded
-#pragma unroll
-                for (int u = 0; u < AS; ++u) {
-                    float2* dst = (p % 2 == 0) ? pa[u] : pb[u];
-#pragma unroll
-                    for (int q = 0; q < RQ; ++q) {
-                        const int b = t + q * TPA;
-                        float2* wp = dst + pad_idx((b / ns) * ns * R + (b & (ns - 1)));
-#pragma unroll
-                        for (int m = 0; m < R; ++m) {
-                            const int e = q + RQ * m;
-                            // base has no bits where m*ns has any: the padded
-                            // index splits into base and a constant
-                            wp[pad_slots(m * ns)] = make_float2(v[u][e].x, v[u][e].y);
-                        }
-                    }
-                }
-                __syncthreads();
-            }
-        }
+        stockham_inverse<N, AS>(v, pa, pb, tw, rbase);
         // after the last pass v[u][e] holds X[t + e*TPA]; e < 4 are the kept bins
 
 #pragma unroll
@@ -582,7 +637,7 @@ chain_kernel(ChainArgs A) {
                             mn = fminf(mn, val[o]);
                             mx = fmaxf(mx, val[o]);
                         }
-                        ascan_minmax<TPA, G::WG>(mn, mx, mm_part + 2 * u * NW, mm_part + (2 * u + 1) * NW);
+                        ascan_minmax<TPA>(mn, mx, mm_part + 2 * u * NW, mm_part + (2 * u + 1) * NW);
                         float range = mx - mn;
                         if (range < 1e-6f) range = 1.0f;
 #pragma unroll
@@ -596,7 +651,7 @@ chain_kernel(ChainArgs A) {
                     for (int o = 0; o < 4; ++o) res[o] = fmaf(A.ep_a, __fsqrt_rn(m2[o]), A.ep_b);
                 }
                 if (active) {
-                    float* dst = A.out + a * G::HALF + t;
+                    OPE_GLOBAL float* dst = gmem(A.out) + a * G::HALF + t;
 #pragma unroll
                     for (int o = 0; o < 4; ++o) {
                         const float r = (flags & F_BG) ? res[o] - bgv[o] : res[o];
@@ -621,6 +676,37 @@ chain_kernel(ChainArgs A) {
             }
         }
     }
+}
+
+// One launch over a frame: persistent workgroups (the grid is sized to the
+// resident capacity) stride over the A-scans; in M_STATS mode each
+// workgroup takes one contiguous chunk so that segment runs are long.
+template <int N, int T, int MODE, int IW, int AS = 1>
+__global__ void __launch_bounds__(Geo<N>::WG) __attribute__((amdgpu_waves_per_eu(OPE_WAVES_PER_EU)))
+chain_kernel(ChainArgs A) {
+    constexpr int WG = Geo<N>::WG;
+    using L = ChainLds<N, MODE, IW, AS, WG>;
+    __shared__ float2 bufA[L::BUF];
+    __shared__ float2 bufB[L::BUF];
+    __shared__ float xsep[L::X];
+    __shared__ typename L::DcSum scan_part[L::SCAN];
+    __shared__ float mm_part[L::MM];
+    constexpr int SLOTS = L::SLOTS;
+    long long a_begin, a_step, a_end;
+    if constexpr ((MODE & M_STATS) != 0) {
+        const long long n_work = A.stat_rows;
+        const long long groups = (n_work + SLOTS - 1) / SLOTS;
+        const long long per = (groups + gridDim.x - 1) / gridDim.x;
+        a_begin = (long long)blockIdx.x * per * SLOTS;
+        a_end = a_begin + per * SLOTS;
+        if (a_end > n_work) a_end = n_work;
+        a_step = SLOTS;
+    } else {
+        a_begin = (long long)blockIdx.x * SLOTS;
+        a_end = A.n_ascans;
+        a_step = (long long)gridDim.x * SLOTS;
+    }
+    chain_body<N, T, MODE, IW, AS, WG>(A, bufA, bufB, xsep, scan_part, mm_part, a_begin, a_end, a_step);
 }
 
 }  // namespace ope_hip

@@ -89,6 +89,38 @@ struct ope_hip_ctx {
     // resident workgroups per CU of each chain kernel used so far (the
     // occupancy query is a runtime call; frames reuse one or two kernels)
     std::vector<std::pair<const void*, int>> occupancy;
+
+    // autofocus: window used by the differentiable chain, FP64 sums
+    float* d_af_window = nullptr;
+    double* d_af_acc = nullptr;
+};
+
+// A ragged batch: snapshot of the class arguments, the task list and the
+// self-resetting counters of ragged_kernel.
+struct ope_hip_ragged {
+    int device = 0;
+    std::string err;
+    int n_reqs = 0;
+    int n_tasks = 0;
+    int grid = 0;
+    RaggedFn fn = nullptr;
+    double tick_hz = 0.0;
+    std::vector<int> req_tasks;  // tasks per request
+    ChainArgs* d_cls = nullptr;
+    RaggedRequest* d_reqs = nullptr;
+    RaggedTask* d_tasks = nullptr;
+    unsigned* d_counters = nullptr;
+    unsigned* d_remaining = nullptr;
+    unsigned long long* d_finish = nullptr;
+    unsigned long long* d_start = nullptr;
+};
+
+// Frames of one context captured into a HIP graph.
+struct ope_hip_graph {
+    int device = 0;
+    std::string err;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
 };
 
 namespace {
@@ -436,7 +468,8 @@ void free_ctx(ope_hip_ctx* c) {
         if (s.d2h_done) (void)hipEventDestroy(s.d2h_done);
     }
     for (void* p : {(void*)c->d_tap_base, (void*)c->d_tap_w, (void*)c->d_phasor, (void*)c->d_twiddle,
-                    (void*)c->d_fpn, (void*)c->d_bg, (void*)c->d_acc})
+                    (void*)c->d_fpn, (void*)c->d_bg, (void*)c->d_acc, (void*)c->d_af_window,
+                    (void*)c->d_af_acc})
         if (p) (void)hipFree(p);
     for (hipStream_t s : {c->s_in, c->s_comp, c->s_out})
         if (s) (void)hipStreamDestroy(s);
@@ -458,6 +491,200 @@ int sync_all(ope_hip_ctx* c) {
     HIP_TRY(c, hipStreamSynchronize(c->s_in));
     HIP_TRY(c, hipStreamSynchronize(c->s_comp));
     HIP_TRY(c, hipStreamSynchronize(c->s_out));
+    return OPE_HIP_OK;
+}
+
+AutofocusLaunch pick_autofocus(int n, int taps, int iw, bool dc, bool bwd) {
+    switch (n) {
+        case 256: return pick_autofocus_n<256>(taps, iw, dc, bwd);
+        case 512: return pick_autofocus_n<512>(taps, iw, dc, bwd);
+        case 1024: return pick_autofocus_n<1024>(taps, iw, dc, bwd);
+        case 2048: return pick_autofocus_n<2048>(taps, iw, dc, bwd);
+        default: return pick_autofocus_n<4096>(taps, iw, dc, bwd);
+    }
+}
+
+// One pass of the differentiable chain over d_in; copies the `n_out` FP64
+// sums to host_out and waits for them.  theta = (d2, d3, c0, c1, c2, c3).
+int autofocus_pass(ope_hip_ctx* c, const void* d_in, long long n_ascans, const double* theta, bool bwd,
+                   float inv_s, float log_inv_s_plus_l, hipStream_t s, double* host_out) {
+    const ope_hip_params& p = c->p;
+    if (!p.resampling_enabled || (p.interpolation != OPE_HIP_INTERP_LINEAR && p.interpolation != OPE_HIP_INTERP_CUBIC))
+        return fail(c, OPE_HIP_EUNSUP, "autofocus needs linear or cubic resampling enabled");
+    const int iw = (int)input_bytes(p.input_type);
+    if (iw == 1) return fail(c, OPE_HIP_EUNSUP, "autofocus supports 16- and 32-bit input");
+    if (n_ascans <= 0 || !d_in) return fail(c, OPE_HIP_EINVAL, "no input");
+    if (reinterpret_cast<uintptr_t>(d_in) % 16) return fail(c, OPE_HIP_EINVAL, "d_in must be 16-byte aligned");
+    if (!theta) return fail(c, OPE_HIP_EINVAL, "theta is null");
+    const int n = c->n;
+    HIP_TRY(c, hipSetDevice(c->device));
+    if (!c->d_af_window) HIP_TRY(c, hipMalloc(&c->d_af_window, n * sizeof(float)));
+    if (!c->d_af_acc) HIP_TRY(c, hipMalloc(&c->d_af_acc, 8 * sizeof(double)));
+    std::vector<float> win(n, 1.0f);
+    if (p.windowing_enabled) win = c->curve_window;
+    HIP_TRY(c, hipMemcpyAsync(c->d_af_window, win.data(), n * sizeof(float), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(fill_zero_f64, dim3(1), dim3(64), 0, s, c->d_af_acc, 8LL);
+    HIP_TRY(c, hipGetLastError());
+    AutofocusArgs a{};
+    a.in = d_in;
+    a.n_ascans = n_ascans;
+    a.window = c->d_af_window;
+    a.twiddle = c->d_twiddle;
+    a.in_type = input_class(p.input_type);
+    a.dc_window = p.dc_window_size;
+    a.d2 = (float)theta[0];
+    a.d3 = (float)theta[1];
+    a.c0 = theta[2];
+    a.c1 = theta[3];
+    a.c2 = theta[4];
+    a.c3 = theta[5];
+    a.inv_s = inv_s;
+    a.log_inv_s_plus_l = log_inv_s_plus_l;
+    a.acc = c->d_af_acc;
+    const AutofocusLaunch l = pick_autofocus(n, c->taps, iw, p.dc_removal_enabled != 0, bwd);
+    int per_cu = 0;
+    HIP_TRY(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(l.fn), l.wg, 0));
+    if (per_cu < 1) per_cu = 1;
+    const long long groups = (n_ascans + l.slots - 1) / l.slots;
+    const long long grid = std::min<long long>(groups, (long long)per_cu * c->cu_count);
+    hipLaunchKernelGGL(l.fn, dim3((unsigned)grid), dim3(l.wg), 0, s, a);
+    HIP_TRY(c, hipGetLastError());
+    HIP_TRY(c, hipMemcpyAsync(host_out, c->d_af_acc, (bwd ? 6 : 2) * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIP_TRY(c, hipStreamSynchronize(s));
+    return OPE_HIP_OK;
+}
+
+void free_ragged(ope_hip_ragged* r) {
+    if (!r) return;
+    (void)hipSetDevice(r->device);
+    for (void* q : {(void*)r->d_cls, (void*)r->d_reqs, (void*)r->d_tasks, (void*)r->d_counters,
+                    (void*)r->d_remaining, (void*)r->d_finish, (void*)r->d_start})
+        if (q) (void)hipFree(q);
+    delete r;
+}
+
+int ragged_fail(ope_hip_ragged* r, int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    r->err = buf;
+    return code;
+}
+
+// Chain arguments of a context for frames that need no FPN statistics or
+// background recording (the flags run_frame sets in that state).
+ChainArgs steady_args(ope_hip_ctx* c) {
+    ChainArgs a = base_args(c);
+    if (c->p.fpn_removal && c->have_fpn) a.flags |= F_FPN;
+    if (c->p.bg_removal && c->have_bg) a.flags |= F_BG;
+    return a;
+}
+
+int build_ragged(ope_hip_ragged* r, ope_hip_ctx* const* ctxs, int n_ctx, const ope_hip_request* reqs,
+                 int n_reqs, int chunk_ascans) {
+    if (!ctxs || n_ctx < 1 || n_ctx > 5) return ragged_fail(r, OPE_HIP_EINVAL, "n_ctx must be in 1..5");
+    if (n_reqs < 0 || (n_reqs > 0 && !reqs)) return ragged_fail(r, OPE_HIP_EINVAL, "bad request array");
+    for (int i = 0; i < n_ctx; ++i) {
+        ope_hip_ctx* c = ctxs[i];
+        if (!c) return ragged_fail(r, OPE_HIP_EINVAL, "context %d is null", i);
+        const ope_hip_ctx* c0 = ctxs[0];
+        if (c->device != c0->device || c->taps != c0->taps ||
+            c->p.dc_removal_enabled != c0->p.dc_removal_enabled ||
+            input_bytes(c->p.input_type) != input_bytes(c0->p.input_type) ||
+            input_class(c->p.input_type) != input_class(c0->p.input_type))
+            return ragged_fail(r, OPE_HIP_EINVAL,
+                               "contexts must share device, interpolation, DC removal and input type");
+        for (int j = 0; j < i; ++j)
+            if (ctxs[j]->n == c->n) return ragged_fail(r, OPE_HIP_EINVAL, "two contexts of length %d", c->n);
+        if ((c->fpn_requested || c->p.fpn_continuous) && c->p.fpn_removal)
+            return ragged_fail(r, OPE_HIP_ESTATE, "context of length %d has an FPN determination pending", c->n);
+        if (c->bg_requested && c->p.bg_removal)
+            return ragged_fail(r, OPE_HIP_ESTATE, "context of length %d has a background recording pending",
+                               c->n);
+    }
+    const ope_hip_ctx* c0 = ctxs[0];
+    r->device = c0->device;
+    r->n_reqs = n_reqs;
+    const size_t iw = input_bytes(c0->p.input_type);
+    std::vector<RaggedRequest> hreq(n_reqs);
+    std::vector<RaggedTask> tasks;
+    r->req_tasks.assign(n_reqs, 0);
+    for (int q = 0; q < n_reqs; ++q) {
+        const ope_hip_request& rq = reqs[q];
+        int cls = -1;
+        for (int i = 0; i < n_ctx; ++i)
+            if (ctxs[i]->n == rq.signal_length) cls = i;
+        if (cls < 0) return ragged_fail(r, OPE_HIP_EINVAL, "request %d: no context of length %d", q, rq.signal_length);
+        if (rq.n_ascans < 0) return ragged_fail(r, OPE_HIP_EINVAL, "request %d: n_ascans < 0", q);
+        if (rq.n_ascans > 0 && (!rq.d_in || !rq.d_out))
+            return ragged_fail(r, OPE_HIP_EINVAL, "request %d: null device pointer", q);
+        if (reinterpret_cast<uintptr_t>(rq.d_in) % 16 || reinterpret_cast<uintptr_t>(rq.d_out) % 4)
+            return ragged_fail(r, OPE_HIP_EINVAL, "request %d: d_in must be 16-byte and d_out 4-byte aligned", q);
+        hreq[q] = RaggedRequest{rq.d_in, rq.d_out, rq.n_ascans};
+        const int n = rq.signal_length;
+        const long long chunk = chunk_ascans > 0 ? chunk_ascans : std::max(1, (1 << 18) / n);
+        const long long nt = (rq.n_ascans + chunk - 1) / chunk;
+        if ((long long)tasks.size() + nt > (1LL << 30))
+            return ragged_fail(r, OPE_HIP_EINVAL, "too many tasks; raise chunk_ascans");
+        for (long long k = 0; k < nt; ++k) {
+            const long long a0 = k * chunk, a1 = std::min<long long>(rq.n_ascans, a0 + chunk);
+            tasks.push_back(RaggedTask{q, cls, n, (int)nt, a0, a1});
+        }
+        r->req_tasks[q] = (int)nt;
+    }
+    r->n_tasks = (int)tasks.size();
+    std::vector<ChainArgs> cls(n_ctx);
+    for (int i = 0; i < n_ctx; ++i) cls[i] = steady_args(ctxs[i]);
+    std::vector<unsigned> remaining(std::max(n_reqs, 1), 0u);
+    for (int q = 0; q < n_reqs; ++q) remaining[q] = (unsigned)r->req_tasks[q];
+
+#define RAGGED_TRY(expr)                                                                         \
+    do {                                                                                         \
+        hipError_t e_ = (expr);                                                                  \
+        if (e_ != hipSuccess)                                                                    \
+            return ragged_fail(r, e_ == hipErrorOutOfMemory ? OPE_HIP_ENOMEM : OPE_HIP_ERUNTIME,  \
+                               "%s: %s", #expr, hipGetErrorString(e_));                          \
+    } while (0)
+    RAGGED_TRY(hipSetDevice(r->device));
+    const size_t nq = (size_t)std::max(n_reqs, 1), ntk = (size_t)std::max(r->n_tasks, 1);
+    RAGGED_TRY(hipMalloc(&r->d_cls, n_ctx * sizeof(ChainArgs)));
+    RAGGED_TRY(hipMalloc(&r->d_reqs, nq * sizeof(RaggedRequest)));
+    RAGGED_TRY(hipMalloc(&r->d_tasks, ntk * sizeof(RaggedTask)));
+    RAGGED_TRY(hipMalloc(&r->d_counters, 2 * sizeof(unsigned)));
+    RAGGED_TRY(hipMalloc(&r->d_remaining, nq * sizeof(unsigned)));
+    RAGGED_TRY(hipMalloc(&r->d_finish, nq * sizeof(unsigned long long)));
+    RAGGED_TRY(hipMalloc(&r->d_start, sizeof(unsigned long long)));
+    RAGGED_TRY(hipMemcpy(r->d_cls, cls.data(), n_ctx * sizeof(ChainArgs), hipMemcpyHostToDevice));
+    if (n_reqs > 0)
+        RAGGED_TRY(hipMemcpy(r->d_reqs, hreq.data(), n_reqs * sizeof(RaggedRequest), hipMemcpyHostToDevice));
+    if (r->n_tasks > 0)
+        RAGGED_TRY(hipMemcpy(r->d_tasks, tasks.data(), tasks.size() * sizeof(RaggedTask), hipMemcpyHostToDevice));
+    const unsigned zero2[2] = {0u, 0u};
+    RAGGED_TRY(hipMemcpy(r->d_counters, zero2, sizeof(zero2), hipMemcpyHostToDevice));
+    RAGGED_TRY(hipMemcpy(r->d_remaining, remaining.data(), nq * sizeof(unsigned), hipMemcpyHostToDevice));
+    std::vector<unsigned long long> zt(nq, 0ull);
+    RAGGED_TRY(hipMemcpy(r->d_finish, zt.data(), nq * sizeof(unsigned long long), hipMemcpyHostToDevice));
+    RAGGED_TRY(hipMemcpy(r->d_start, zt.data(), sizeof(unsigned long long), hipMemcpyHostToDevice));
+
+    RaggedFn fn = nullptr;
+    const int mode = c0->p.dc_removal_enabled ? M_DC : 0;
+    switch (c0->taps) {
+        case 1: fn = pick_ragged_t<1>(mode, (int)iw); break;
+        case 2: fn = pick_ragged_t<2>(mode, (int)iw); break;
+        case 4: fn = pick_ragged_t<4>(mode, (int)iw); break;
+        default: fn = pick_ragged_t<16>(mode, (int)iw); break;
+    }
+    r->fn = fn;
+    int per_cu = 0;
+    RAGGED_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(fn), kRaggedWG, 0));
+    if (per_cu < 1) per_cu = 1;
+    r->grid = (int)std::min<long long>((long long)per_cu * c0->cu_count, std::max(r->n_tasks, 1));
+    int khz = 0;
+    RAGGED_TRY(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, r->device));
+    r->tick_hz = (double)khz * 1e3;
+#undef RAGGED_TRY
     return OPE_HIP_OK;
 }
 
@@ -728,6 +955,8 @@ int ope_hip_process_device(ope_hip_ctx* c, const void* d_in, float* d_out, int64
     if (!c) return OPE_HIP_EINVAL;
     if (n_ascans < 0) return fail(c, OPE_HIP_EINVAL, "n_ascans must be >= 0");
     if (n_ascans > 0 && (!d_in || !d_out)) return fail(c, OPE_HIP_EINVAL, "null device pointer");
+    if (reinterpret_cast<uintptr_t>(d_in) % 16 || reinterpret_cast<uintptr_t>(d_out) % 4)
+        return fail(c, OPE_HIP_EINVAL, "d_in must be 16-byte and d_out 4-byte aligned");
     std::lock_guard<std::mutex> lk(c->submit_mu);
     HIP_TRY(c, hipSetDevice(c->device));
     hipStream_t s = static_cast<hipStream_t>(stream);  // NULL: the HIP null stream
@@ -826,6 +1055,193 @@ int ope_hip_host_alloc(size_t bytes, void** ptr) {
 
 int ope_hip_host_free(void* ptr) {
     if (ptr) (void)hipHostFree(ptr);
+    return OPE_HIP_OK;
+}
+
+int ope_hip_autofocus_loss(ope_hip_ctx* c, const void* d_in, int64_t n_ascans, const double* theta, double* out,
+                           void* stream) {
+    if (!c || !out) return OPE_HIP_EINVAL;
+    std::lock_guard<std::mutex> lk(c->submit_mu);
+    double acc[2] = {0.0, 0.0};
+    const int rc = autofocus_pass(c, d_in, n_ascans, theta, false, 0.f, 0.f, static_cast<hipStream_t>(stream), acc);
+    if (rc) return rc;
+    const double S = acc[0], Q = acc[1];
+    out[0] = std::log(S) - Q / S;
+    out[1] = S;
+    out[2] = Q;
+    return OPE_HIP_OK;
+}
+
+int ope_hip_autofocus_grad(ope_hip_ctx* c, const void* d_in, int64_t n_ascans, const double* theta, double sum_i,
+                           double loss, double* grad, void* stream) {
+    if (!c || !grad) return OPE_HIP_EINVAL;
+    if (!(sum_i > 0.0)) return fail(c, OPE_HIP_EINVAL, "sum_i must be > 0");
+    std::lock_guard<std::mutex> lk(c->submit_mu);
+    const double inv_s = 1.0 / sum_i;
+    return autofocus_pass(c, d_in, n_ascans, theta, true, (float)inv_s, (float)(std::log(inv_s) + loss),
+                          static_cast<hipStream_t>(stream), grad);
+}
+
+int ope_hip_graph_create(ope_hip_ctx* c, const void* const* d_in, float* const* d_out, int n_frames,
+                         int64_t ascans_per_frame, ope_hip_graph** out) {
+    g_create_err.clear();
+    if (!out) {
+        g_create_err = "out is null";
+        return OPE_HIP_EINVAL;
+    }
+    *out = nullptr;
+    if (!c || !d_in || !d_out || n_frames < 1 || ascans_per_frame < 1) {
+        g_create_err = "bad arguments";
+        return OPE_HIP_EINVAL;
+    }
+    std::lock_guard<std::mutex> lk(c->submit_mu);
+    const ope_hip_params& p = c->p;
+    // steady state: the frames would launch the chain kernel only
+    const long long required = (long long)p.fpn_bscan_count * p.ascans_per_bscan;
+    const long long rows_used = (required / kFpnSegments) * kFpnSegments;
+    if (c->fpn_requested || p.fpn_continuous) {
+        g_create_err = "an FPN determination is pending";
+        return OPE_HIP_ESTATE;
+    }
+    if (rows_used > 0 && c->fpn_count < rows_used) {
+        g_create_err = "FPN statistics are still accumulating: process frames first";
+        return OPE_HIP_ESTATE;
+    }
+    if (c->bg_requested && p.bg_removal) {
+        g_create_err = "a background recording is pending";
+        return OPE_HIP_ESTATE;
+    }
+    for (int f = 0; f < n_frames; ++f) {
+        if (!d_in[f] || !d_out[f] || reinterpret_cast<uintptr_t>(d_in[f]) % 16 ||
+            reinterpret_cast<uintptr_t>(d_out[f]) % 4) {
+            g_create_err = "frame buffers must be non-null, d_in 16-byte and d_out 4-byte aligned";
+            return OPE_HIP_EINVAL;
+        }
+    }
+    ope_hip_graph* g = new (std::nothrow) ope_hip_graph();
+    if (!g) {
+        g_create_err = "out of host memory";
+        return OPE_HIP_ENOMEM;
+    }
+    g->device = c->device;
+    hipStream_t s = nullptr;
+    auto cleanup = [&](int code, const char* what, hipError_t e) {
+        char buf[256];
+        snprintf(buf, sizeof(buf), "%s: %s", what, hipGetErrorString(e));
+        g_create_err = buf;
+        if (s) (void)hipStreamDestroy(s);
+        if (g->exec) (void)hipGraphExecDestroy(g->exec);
+        if (g->graph) (void)hipGraphDestroy(g->graph);
+        delete g;
+        return code;
+    };
+    hipError_t e = hipSetDevice(c->device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+    if (e != hipSuccess) return cleanup(OPE_HIP_ERUNTIME, "stream", e);
+    e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return cleanup(OPE_HIP_ERUNTIME, "hipStreamBeginCapture", e);
+    // the frame bookkeeping of run_frame is host state: keep it as it was
+    const long long fpn_count = c->fpn_count;
+    int rc = OPE_HIP_OK;
+    for (int f = 0; f < n_frames && rc == OPE_HIP_OK; ++f) rc = run_frame(c, d_in[f], d_out[f], ascans_per_frame, s);
+    c->fpn_count = fpn_count;
+    e = hipStreamEndCapture(s, &g->graph);
+    if (rc != OPE_HIP_OK) {
+        const std::string msg = c->err;
+        cleanup(rc, "capture", hipSuccess);
+        g_create_err = msg;
+        return rc;
+    }
+    if (e != hipSuccess) return cleanup(OPE_HIP_ERUNTIME, "hipStreamEndCapture", e);
+    e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) return cleanup(OPE_HIP_ERUNTIME, "hipGraphInstantiate", e);
+    (void)hipStreamDestroy(s);
+    *out = g;
+    return OPE_HIP_OK;
+}
+
+int ope_hip_graph_launch(ope_hip_graph* g, void* stream) {
+    if (!g) return OPE_HIP_EINVAL;
+    hipError_t e = hipSetDevice(g->device);
+    if (e == hipSuccess) e = hipGraphLaunch(g->exec, static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) {
+        g->err = std::string("hipGraphLaunch: ") + hipGetErrorString(e);
+        return OPE_HIP_ERUNTIME;
+    }
+    return OPE_HIP_OK;
+}
+
+const char* ope_hip_graph_last_error(const ope_hip_graph* g) { return g ? g->err.c_str() : g_create_err.c_str(); }
+
+int ope_hip_graph_destroy(ope_hip_graph* g) {
+    if (!g) return OPE_HIP_EINVAL;
+    (void)hipSetDevice(g->device);
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    delete g;
+    return OPE_HIP_OK;
+}
+
+int ope_hip_ragged_create(ope_hip_ctx* const* ctxs, int n_ctx, const ope_hip_request* reqs, int n_reqs,
+                          int chunk_ascans, ope_hip_ragged** out) {
+    g_create_err.clear();
+    if (!out) {
+        g_create_err = "out is null";
+        return OPE_HIP_EINVAL;
+    }
+    *out = nullptr;
+    ope_hip_ragged* r = new (std::nothrow) ope_hip_ragged();
+    if (!r) {
+        g_create_err = "out of host memory";
+        return OPE_HIP_ENOMEM;
+    }
+    const int rc = build_ragged(r, ctxs, n_ctx, reqs, n_reqs, chunk_ascans);
+    if (rc) {
+        g_create_err = r->err;
+        free_ragged(r);
+        return rc;
+    }
+    *out = r;
+    return OPE_HIP_OK;
+}
+
+int ope_hip_ragged_run(ope_hip_ragged* r, void* stream) {
+    if (!r) return OPE_HIP_EINVAL;
+    if (r->n_tasks == 0) return OPE_HIP_OK;
+    hipError_t e = hipSetDevice(r->device);
+    if (e != hipSuccess) return ragged_fail(r, OPE_HIP_ERUNTIME, "hipSetDevice: %s", hipGetErrorString(e));
+    RaggedArgs a{r->d_cls, r->d_reqs, r->d_tasks, r->n_tasks, r->d_counters, r->d_remaining, r->d_finish,
+                 r->d_start};
+    hipLaunchKernelGGL(r->fn, dim3((unsigned)r->grid), dim3(kRaggedWG), 0, static_cast<hipStream_t>(stream), a);
+    e = hipGetLastError();
+    if (e != hipSuccess) return ragged_fail(r, OPE_HIP_ERUNTIME, "ragged launch: %s", hipGetErrorString(e));
+    return OPE_HIP_OK;
+}
+
+int ope_hip_ragged_latencies(ope_hip_ragged* r, double* latency_us) {
+    if (!r || (!latency_us && r->n_reqs > 0)) return OPE_HIP_EINVAL;
+    if (r->n_reqs == 0) return OPE_HIP_OK;
+    std::vector<unsigned long long> fin(r->n_reqs);
+    unsigned long long start = 0;
+    hipError_t e = hipSetDevice(r->device);
+    if (e == hipSuccess) e = hipMemcpy(fin.data(), r->d_finish, fin.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(&start, r->d_start, sizeof(start), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return ragged_fail(r, OPE_HIP_ERUNTIME, "latencies: %s", hipGetErrorString(e));
+    for (int q = 0; q < r->n_reqs; ++q) {
+        if (r->req_tasks[q] == 0 || fin[q] < start) {
+            latency_us[q] = 0.0;
+            continue;
+        }
+        latency_us[q] = (double)(fin[q] - start) / r->tick_hz * 1e6;
+    }
+    return OPE_HIP_OK;
+}
+
+const char* ope_hip_ragged_last_error(const ope_hip_ragged* r) { return r ? r->err.c_str() : g_create_err.c_str(); }
+
+int ope_hip_ragged_destroy(ope_hip_ragged* r) {
+    if (!r) return OPE_HIP_EINVAL;
+    free_ragged(r);
     return OPE_HIP_OK;
 }
 

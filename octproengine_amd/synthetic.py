@@ -6,8 +6,11 @@ Fringe model (per A-scan, N samples):
 with per-A-scan reflector depths z_r ~ U(50, 900) * N/2048, amplitudes
 a_r ~ U(500, 3000), Gaussian noise eps ~ N(0, sigma^2), DC = 20000.  After
 resampling at r[j] = k^-1(j) the fringe of reflector r is a pure tone at
-bin z_r.  ``phi`` is an optional dispersion phase that the compensation
-curve removes.
+bin z_r.  ``phi`` is an optional dispersion phase,
+phi(k) = d2 (k-N/2)^2 + d3 (k-N/2)^3: the chain's dispersion step with the
+same coefficients (phase curve ``dispersion_phase(N, d2, d3)``) removes it.
+(The chain keeps bins k < N/2 of an inverse FFT, i.e. the exp(-i arg)
+half of each cosine, which the compensation multiplies by exp(+i phi).)
 
 numpy (``fringes``) is used for parity-sized data; ``fringes_torch`` makes
 the same kind of data directly in GPU memory for throughput runs.
@@ -58,7 +61,7 @@ def fringes(n: int, n_ascans: int, seed: int = 0, reflectors: int = 3, sigma: fl
     if d2 or d3:
         # dispersion acts on the linear-k axis, so it is evaluated at k(n)
         kc = kk - n / 2
-        phi = -(d2 * kc * kc + d3 * kc * kc * kc)
+        phi = d2 * kc * kc + d3 * kc * kc * kc
     out = np.empty((n_ascans, n), dtype=np.float64)
     chunk = max(1, (1 << 22) // n)
     for s in range(0, n_ascans, chunk):
@@ -73,14 +76,17 @@ def fringes(n: int, n_ascans: int, seed: int = 0, reflectors: int = 3, sigma: fl
 
 
 def fringes_torch(n: int, n_ascans: int, seed: int = 0, reflectors: int = 3, sigma: float = 30.0,
-                  device: str = "cuda", dtype=None, chunk: int = 4096):
+                  device: str = "cuda", dtype=None, chunk: int = 4096, d2: float = 0.0, d3: float = 0.0):
     """Same model as ``fringes`` generated on the GPU with torch (uint16 as
     int16 bit patterns in a torch.int16 tensor, or float32 in [0, 1])."""
     import torch
 
     g = torch.Generator(device=device)
     g.manual_seed(seed)
-    kk = torch.as_tensor(k_of_n(n), device=device, dtype=torch.float32)
+    kk64 = k_of_n(n)
+    kc = kk64 - n / 2
+    kk = torch.as_tensor(kk64, device=device, dtype=torch.float32)
+    phi = torch.as_tensor(d2 * kc * kc + d3 * kc * kc * kc, device=device, dtype=torch.float32)
     want_f32 = dtype == torch.float32
     out = torch.empty((n_ascans, n), device=device,
                       dtype=torch.float32 if want_f32 else torch.int16)
@@ -92,7 +98,7 @@ def fringes_torch(n: int, n_ascans: int, seed: int = 0, reflectors: int = 3, sig
         a = torch.rand((m, reflectors, 1), generator=g, device=device) * 2500.0 + 500.0
         acc = torch.full((m, n), DC, device=device, dtype=torch.float32)
         for r in range(reflectors):
-            acc += (a[:, r] * torch.cos((2.0 * math.pi / n) * z[:, r] * kk[None, :]))
+            acc += (a[:, r] * torch.cos((2.0 * math.pi / n) * z[:, r] * kk[None, :] + phi[None, :]))
         acc += torch.randn((m, n), generator=g, device=device) * sigma
         acc = torch.clamp(torch.round(acc), 0, 65535)
         if want_f32:
