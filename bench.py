@@ -122,14 +122,175 @@ def cpu_baseline(cfg, seconds: float, threads: int):
     return done / dt, done, dt
 
 
+def ragged_workload(seed=0, count=512):
+    """512 requests of signal length 1024, 2048 or 4096 (uniform) and 64..1024
+    A-scans (uniform), float32 input = uint16 fringes / 65535."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    lengths = rng.choice([1024, 2048, 4096], size=count)
+    ascans = rng.integers(64, 1025, size=count)
+    return [(int(n), int(a)) for n, a in zip(lengths, ascans)]
+
+
+def ragged_measurement(capi, S, torch, stream, reps=3):
+    """One persistent launch over the mixed-length batch; A-scans/s of the
+    whole batch and the p50/p99 per-request latency inside the launch, plus
+    the same requests as one ope_hip_process_device launch each."""
+    import numpy as np
+    reqs = ragged_workload()
+    ctxs = {}
+    for n in (1024, 2048, 4096):
+        params = capi.make_params(signal_length=n, ascans_per_bscan=1024, bscans_per_buffer=1,
+                                  input_type=capi.FLOAT32, resampling_enabled=1, interpolation=capi.CUBIC,
+                                  windowing_enabled=1, dispersion_enabled=1, log_scaling=1,
+                                  grayscale_min=30.0, grayscale_max=100.0)
+        c = capi.Context(params)
+        c.set_resampling_curve(S.resampling_curve(n))
+        c.set_window_curve(hann_window(n))
+        c.set_dispersion_curve(cos_sin(S.dispersion_phase(n, 5e-6 * (2048.0 / n) ** 2, -2e-9 * (2048.0 / n) ** 3)))
+        ctxs[n] = c
+    d_in, d_out = [], []
+    for i, (n, a) in enumerate(reqs):
+        d_in.append(S.fringes_torch(n, a, seed=10_000 + i, device="cuda", dtype=torch.float32))
+        d_out.append(torch.empty((a, n // 2), dtype=torch.float32, device="cuda"))
+    total = sum(a for _, a in reqs)
+    batch = capi.RaggedBatch([ctxs[n] for n in (1024, 2048, 4096)],
+                             [(x.data_ptr(), y.data_ptr(), a, n) for x, y, (n, a) in zip(d_in, d_out, reqs)])
+    sp = stream.cuda_stream
+    batch.run(sp)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        batch.run(sp)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    lat = batch.latencies_us()
+    # the same requests, one launch each
+    e0.record(stream)
+    for x, y, (n, a) in zip(d_in, d_out, reqs):
+        ctxs[n].process_device(x.data_ptr(), y.data_ptr(), a, sp)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms_each = e0.elapsed_time(e1)
+    in_bytes = sum(a * n * 4 for n, a in reqs)
+    out_bytes = sum(a * (n // 2) * 4 for n, a in reqs)
+    batch.close()
+    for c in ctxs.values():
+        c.close()
+    return {
+        "workload": "512 requests, N uniform in {1024, 2048, 4096}, 64..1024 A-scans, float32, "
+                    "cubic + Hann + dispersion + log",
+        "ascans": total, "value": round(total / (ms / 1e3), 1), "unit": "A-scans/s",
+        "ms_per_batch": round(ms, 3), "hbm_gb_s": round((in_bytes + out_bytes) / (ms / 1e3) / 1e9, 1),
+        "latency_p50_ms": round(float(np.percentile(lat, 50)) / 1e3, 3),
+        "latency_p99_ms": round(float(np.percentile(lat, 99)) / 1e3, 3),
+        "one_launch_per_request": {"value": round(total / (ms_each / 1e3), 1), "unit": "A-scans/s"},
+    }
+
+
+def volume_measurement(capi, S, torch, stream, reps=3):
+    """A 2048 x 1024 x 256 uint16 volume (8 reflectors per A-scan), Lanczos
+    resampling + Hann + dispersion + DC removal + log, as 16 frames of
+    16 B-scans: replayed from one captured HIP graph, and as one launch over
+    the whole volume."""
+    n, frame, frames = 2048, 16 * 1024, 16
+    params = capi.make_params(signal_length=n, ascans_per_bscan=1024, bscans_per_buffer=16, input_type=capi.UINT16,
+                              resampling_enabled=1, interpolation=capi.LANCZOS, windowing_enabled=1,
+                              dispersion_enabled=1, dc_removal_enabled=1, dc_window_size=64, log_scaling=1,
+                              grayscale_min=30.0, grayscale_max=100.0)
+    ctx = capi.Context(params)
+    ctx.set_resampling_curve(S.resampling_curve(n))
+    ctx.set_window_curve(hann_window(n))
+    ctx.set_dispersion_curve(cos_sin(S.dispersion_phase(n, 5e-6, -2e-9)))
+    vol = S.fringes_torch(n, frame * frames, seed=77, reflectors=8, device="cuda")
+    out = torch.empty((frame * frames, n // 2), dtype=torch.float32, device="cuda")
+    sp = stream.cuda_stream
+    ctx.process_device(vol[:frame].data_ptr(), out[:frame].data_ptr(), frame, sp)  # FPN statistics pass
+    torch.cuda.synchronize()
+    ins = [vol[f * frame:(f + 1) * frame].data_ptr() for f in range(frames)]
+    outs = [out[f * frame:(f + 1) * frame].data_ptr() for f in range(frames)]
+    graph = capi.FrameGraph(ctx, ins, outs, frame)
+    graph.launch(sp)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        graph.launch(sp)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms_graph = e0.elapsed_time(e1) / reps
+    e0.record(stream)
+    for _ in range(reps):
+        ctx.process_device(vol.data_ptr(), out.data_ptr(), frame * frames, sp)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms_one = e0.elapsed_time(e1) / reps
+    total = frame * frames
+    graph.close()
+    ctx.close()
+    return {"workload": "2048 x 1024 x 256 uint16 volume, 8 reflectors, Lanczos-8 + Hann + dispersion + DC + log",
+            "graph_16_frames": {"value": round(total / (ms_graph / 1e3), 1), "unit": "A-scans/s",
+                                "ms_per_volume": round(ms_graph, 3)},
+            "one_launch": {"value": round(total / (ms_one / 1e3), 1), "unit": "A-scans/s",
+                           "ms_per_volume": round(ms_one, 3)}}
+
+
+def autofocus_measurement(capi, S, torch):
+    """Dispersion autofocus on 2048 x 512 x 8 float32 fringes made with
+    d2 = 8e-6, d3 = -3e-9: time of one loss + gradient evaluation and of 50
+    Adam steps from d2 = d3 = 0 through torch.autograd."""
+    import numpy as np
+    from octproengine_amd.autofocus import autofocus
+    n, a = 2048, 512 * 8
+    params = capi.make_params(signal_length=n, ascans_per_bscan=512, bscans_per_buffer=8, input_type=capi.FLOAT32,
+                              resampling_enabled=1, interpolation=capi.CUBIC, windowing_enabled=1,
+                              dc_removal_enabled=1, dc_window_size=64)
+    ctx = capi.Context(params)
+    ctx.set_window_curve(hann_window(n))
+    frames = S.fringes_torch(n, a, seed=5, device="cuda", dtype=torch.float32, d2=8e-6, d3=-3e-9)
+    r = S.resampling_curve(n, clamp=False).astype(np.float64)
+    u = np.arange(n) / (n - 1)
+    c = np.linalg.lstsq(np.vstack([u ** 0, u, u ** 2, u ** 3]).T, r, rcond=None)[0]
+    theta = np.r_[0.0, 0.0, c]
+    ptr = frames.data_ptr()
+    for _ in range(2):
+        loss, s_i, _ = ctx.autofocus_loss(ptr, a, theta)
+        ctx.autofocus_grad(ptr, a, theta, s_i, loss)
+    t0 = time.perf_counter()
+    reps = 10
+    for _ in range(reps):
+        loss, s_i, _ = ctx.autofocus_loss(ptr, a, theta)
+        ctx.autofocus_grad(ptr, a, theta, s_i, loss)
+    ms_eval = (time.perf_counter() - t0) / reps * 1e3
+    scale = np.r_[1e-5, 1e-9, 1, 1, 1, 1]
+    free = (True, True, False, False, False, False)
+    autofocus(ctx, frames, theta, steps=1, lr=0.1, scale=scale, free=free)  # optimizer set-up
+    t0 = time.perf_counter()
+    th, hist = autofocus(ctx, frames, theta, steps=50, lr=0.1, scale=scale, free=free)
+    ms_adam = (time.perf_counter() - t0) * 1e3
+    ctx.close()
+    return {"workload": "2048 x 512 x 8 float32, cubic + Hann + DC, entropy of |X|^2; true d2 = 8e-6, d3 = -3e-9",
+            "ms_per_loss_and_gradient": round(ms_eval, 3),
+            "adam_50_steps_ms": round(ms_adam, 1), "loss_first_last": [round(hist[0], 4), round(hist[-1], 4)],
+            "d2_d3_found": [float(f"{th[0]:.4e}"), float(f"{th[1]:.4e}")]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--ring", type=int, default=8, help="distinct frames cycled through")
+    ap.add_argument("--ramp-seconds", type=float, default=1.0,
+                    help="untimed back-to-back frames before the warm-up steps, so that the GPU has left "
+                         "its idle clock state (the timed region measures steady-state throughput)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-ragged", action="store_true", help="skip the mixed-length batch measurement")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the volume (graph) and autofocus measurements")
     ap.add_argument("--host-path", action="store_true",
                     help="also time host frames through ope_hip_submit_host (pinned host buffers, "
                          "H2D + chain + D2H): the PCIe-inclusive rate, reported beside the value")
@@ -165,7 +326,17 @@ def main():
     def step(i):
         ctx.process_device(ins[i % args.ring].data_ptr(), outs[i % args.ring].data_ptr(), frame, sp)
 
-    for i in range(max(args.warmup, 2)):  # >= 2: FPN determined + re-accumulated
+    for i in range(2):  # FPN determined, then the next determination's statistics
+        step(i)
+    torch.cuda.synchronize()
+    t_ramp = time.perf_counter()
+    ramp_frames = 0
+    while time.perf_counter() - t_ramp < args.ramp_seconds:
+        for _ in range(16):
+            step(ramp_frames)
+            ramp_frames += 1
+        torch.cuda.synchronize()
+    for i in range(args.warmup):
         step(i)
     torch.cuda.synchronize()
     ev0 = torch.cuda.Event(enable_timing=True)
@@ -194,6 +365,10 @@ def main():
     ev1.record(stream)
     torch.cuda.synchronize()
     batched = 4 * frame * reps / (ev0.elapsed_time(ev1) / 1e3)
+
+    ragged = None if args.no_ragged else ragged_measurement(capi, S, torch, stream)
+    volume = None if args.no_extras else volume_measurement(capi, S, torch, stream)
+    focus = None if args.no_extras else autofocus_measurement(capi, S, torch)
 
     host_rate = None
     if args.host_path:
@@ -259,8 +434,11 @@ def main():
         "data": "synthetic (seeded fringes generated on the GPU)",
         "config": {"workload": cfg["workload"], "signal_length": n,
                    "ascans_per_frame": frame, "input": "uint16", "frames_in_ring": args.ring,
-                   "wall_s_timed_region": round(wall, 4)},
+                   "wall_s_timed_region": round(wall, 4), "clock_ramp_frames_untimed": ramp_frames},
         "batched_4_frames_per_launch": {"value": round(batched, 1), "unit": "A-scans/s"},
+        "ragged_batch": ragged,
+        "volume": volume,
+        "autofocus": focus,
         "host_buffers_pcie_inclusive": (None if host_rate is None else
                                         {"value": round(host_rate, 1), "unit": "A-scans/s",
                                          "path": "ope_hip_submit_host, 2 pinned frames in flight"}),

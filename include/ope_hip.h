@@ -86,6 +86,16 @@ typedef struct ope_hip_params {
     int32_t bg_removal;         /* post-process background subtraction    */
     float bg_weight;
     float bg_offset;
+    /* The CUDA backend's extra steps (the CPU backend ignores them):
+     * bitshift: samples >> 4 (8/16-bit) or * 2^-32 (32-bit integer input)
+     *   before conversion (cuda_kernels.cu:45-70); no effect on FLOAT32;
+     * bscan_flip: reverse the A-scan order of B-scans 0, 2, 4, ... of a frame
+     *   (cuda_kernels.cu:553-571; the last B-scan of an odd count stays);
+     * sinusoidal_correction: resample each B-scan from sinusoidal to linear
+     *   scan positions (cuda_kernels.cu:408-437). */
+    int32_t bitshift;
+    int32_t bscan_flip;
+    int32_t sinusoidal_correction;
 } ope_hip_params;
 
 typedef struct ope_hip_ctx ope_hip_ctx;

@@ -80,7 +80,8 @@ class Params(Structure):
         ("log_scaling", c_int32), ("grayscale_min", c_float), ("grayscale_max", c_float),
         ("multiplicator", c_float), ("addend", c_float), ("fpn_removal", c_int32),
         ("fpn_bscan_count", c_int32), ("fpn_continuous", c_int32), ("bg_removal", c_int32),
-        ("bg_weight", c_float), ("bg_offset", c_float),
+        ("bg_weight", c_float), ("bg_offset", c_float), ("bitshift", c_int32), ("bscan_flip", c_int32),
+        ("sinusoidal_correction", c_int32),
     ]
 
 
@@ -97,7 +98,7 @@ def make_params(**kw) -> Params:
              dispersion_enabled=0, dc_removal_enabled=0, dc_window_size=64, log_scaling=1,
              grayscale_min=30.0, grayscale_max=100.0, multiplicator=1.0, addend=0.0,
              fpn_removal=0, fpn_bscan_count=1, fpn_continuous=0, bg_removal=0,
-             bg_weight=1.0, bg_offset=0.0)
+             bg_weight=1.0, bg_offset=0.0, bitshift=0, bscan_flip=0, sinusoidal_correction=0)
     unknown = set(kw) - set(d)
     if unknown:
         raise TypeError(f"unknown parameters: {sorted(unknown)}")

@@ -48,6 +48,9 @@ ope_hip_params to_params(const ProcessorConfiguration& c) {
     p.bg_removal = c.postProcessingParams.backgroundRemoval;
     p.bg_weight = c.postProcessingParams.backgroundWeight;
     p.bg_offset = c.postProcessingParams.backgroundOffset;
+    p.bitshift = c.dataParams.bitshift;
+    p.bscan_flip = c.postProcessingParams.bscanFlip;
+    p.sinusoidal_correction = c.postProcessingParams.sinusoidalScanCorrection;
     return p;
 }
 

@@ -33,6 +33,7 @@ struct AutofocusArgs {
     const float* window;     // [N]
     const float2* twiddle;   // [N] exp(+2 pi i k / N)
     int in_type;
+    int bitshift;
     int dc_window;
     double c0, c1, c2, c3;   // resampling polynomial (evaluated in FP64: a float32
                              // position is off by ~1e-4 samples at N = 2048,
@@ -82,7 +83,7 @@ __global__ void __launch_bounds__(Geo<N>::WG) autofocus_kernel(AutofocusArgs A) 
         uint32_t raw[8] = {0, 0, 0, 0, 0, 0, 0, 0};
         if (active) load_raw<IW>(A.in, N, a, t, raw);
         float xv[8];
-        convert_raw<IW>(ieee, raw, xv);
+        convert_raw<IW>(ieee, A.bitshift != 0, raw, xv);
         __syncthreads();  // the previous A-scan's LDS reads are done
         if constexpr (DC)
             dc_remove<N, TPA, typename L::DcSum>(xv, t, A.dc_window, reinterpret_cast<typename L::DcSum*>(pa[0]),

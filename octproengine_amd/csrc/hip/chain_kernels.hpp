@@ -98,7 +98,7 @@ struct Geo {
     static constexpr int WG = TPA >= 256 ? TPA : 256;       // workgroup size of chain_kernel
     static constexpr int APW = WG / TPA;                    // A-scans per chain_kernel workgroup
     static constexpr int NPASS = (LOG2N + 2) / 3;           // Stockham passes
-    static constexpr int LAST_R = N >> (3 * (NPASS - 1));   // radix of the last pass
+    static constexpr int LAST_R = N >> (3 * (NPASS - 1));   // radix of the one non-radix-8 pass
     // padded float2 slots; at least N+1 so buffer A also holds the N+1
     // prefix sums (int or double) of the DC-removal prologue
     static constexpr int PADN = pad_slots(N - 1) + 1 > N + 1 ? pad_slots(N - 1) + 1 : N + 1;
@@ -117,6 +117,8 @@ enum : int {
     F_BG = 8,        // subtract the post-process background
     F_LOG = 16,      // log scaling (else linear)
     F_AUTOMM = 32,   // log scaling with per-A-scan min/max
+    F_BITSHIFT = 64, // input samples >> 4 (8/16-bit) or * 2^-32 (32-bit integers)
+    F_FLIP = 128,    // reverse the A-scan order of B-scans 0, 2, 4, ...
 };
 
 struct ChainArgs {
@@ -136,6 +138,8 @@ struct ChainArgs {
     float coeff, addend;     // auto min/max epilogue
     float log_c, log_h;      // v = log_c * (log2(m2) - log_h) = 10 log10(m2/(N/2))
     float bg_w, bg_off;
+    int ascans_per_bscan;    // B-scan flip: B-scan length
+    long long flip_bscans;   // B-scans b with b even and b + 1 < flip_bscans are flipped
     // FPN statistics mode
     double* acc;             // [8][N/2][3] (sum re, sum im, sum |X|^2)
     long long row0;          // accumulated row index of A-scan 0 of this call
@@ -199,21 +203,34 @@ __device__ __forceinline__ void load_raw(const void* in, int n, long long a, int
 }
 
 // Samples to float (cpu_kernels.tpp:33-52): unsigned integers by width;
-// 4-byte words are IEEE floats when `ieee` is set (FLOAT32 input).
+// 4-byte words are IEEE floats when `ieee` is set (FLOAT32 input).  `shift`
+// is the CUDA backend's bitshift option (cuda_kernels.cu:45-70): 8/16-bit
+// samples >> 4, 32-bit integers * 2^-32 (exact: a power-of-two scaling of
+// the rounded value); IEEE floats are not shifted.
 template <int IW>
-__device__ __forceinline__ void convert_raw(bool ieee, const uint32_t (&raw)[8], float (&x)[8]) {
+__device__ __forceinline__ void convert_raw(bool ieee, bool shift, const uint32_t (&raw)[8], float (&x)[8]) {
     if constexpr (IW == 2) {
+        if (shift) {
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            x[2 * i] = (float)(raw[i] & 0xffffu);
-            x[2 * i + 1] = (float)(raw[i] >> 16);
+            for (int i = 0; i < 4; ++i) {
+                x[2 * i] = (float)((raw[i] & 0xffffu) >> 4);
+                x[2 * i + 1] = (float)(raw[i] >> 20);
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                x[2 * i] = (float)(raw[i] & 0xffffu);
+                x[2 * i + 1] = (float)(raw[i] >> 16);
+            }
         }
     } else if constexpr (IW == 1) {
+        const int s = shift ? 4 : 0;
 #pragma unroll
-        for (int i = 0; i < 8; ++i) x[i] = (float)((raw[i >> 2] >> (8 * (i & 3))) & 0xffu);
+        for (int i = 0; i < 8; ++i) x[i] = (float)(((raw[i >> 2] >> (8 * (i & 3))) & 0xffu) >> s);
     } else {
+        const float scale = (shift && !ieee) ? 2.3283064365386963e-10f : 1.0f;  // 2^-32
 #pragma unroll
-        for (int i = 0; i < 8; ++i) x[i] = ieee ? __uint_as_float(raw[i]) : (float)raw[i];
+        for (int i = 0; i < 8; ++i) x[i] = ieee ? __uint_as_float(raw[i]) : (float)raw[i] * scale;
     }
 }
 
@@ -274,18 +291,37 @@ __device__ __forceinline__ void ascan_minmax(float& mn, float& mx, float* part_m
 
 // ------------------------------------------------------------------ FFT
 
+// Radix schedule of the Stockham passes: every pass is radix 8 except, when
+// N is not a power of 8, one pass of radix N / 8^(NPASS-1) at position
+// SMALL_AT (SCHED < 0: the last pass).  Any position gives the same
+// natural-order result; they differ in twiddle count and LDS access pattern.
+template <int N, int SCHED>
+struct FftPlan {
+    static constexpr int NP = Geo<N>::NPASS;
+    static constexpr int SMALL = Geo<N>::LAST_R;
+    static constexpr int SMALL_AT = (SMALL == 8 || SCHED < 0 || SCHED >= NP) ? NP - 1 : SCHED;
+    static constexpr int radix(int p) { return p == SMALL_AT ? SMALL : 8; }
+    // product of the radices of passes 0..p-1
+    static constexpr int ns(int p) {
+        int v = 1;
+        for (int q = 0; q < p; ++q) v *= radix(q);
+        return v;
+    }
+};
+
 // Twiddles of Stockham passes 1..NPASS-1 for lane t (pass 0 has Ns = 1: all
 // ones); elements with r == 0 need none.  table = exp(+2 pi i k / N).
-template <int N>
+template <int N, int SCHED = -1>
 __device__ __forceinline__ void load_twiddles(const float2* table, int t, cf (&tw)[Geo<N>::NPASS][8]) {
     using G = Geo<N>;
+    using P = FftPlan<N, SCHED>;
     constexpr int NP = G::NPASS;
     constexpr int TPA = G::TPA;
 #pragma unroll
     for (int p = 1; p < NP; ++p) {
-        const int R = (p < NP - 1) ? 8 : G::LAST_R;
+        const int R = P::radix(p);
         const int RQ = 8 / R;
-        const int ns = 1 << (3 * p);
+        const int ns = P::ns(p);
         const int step = N / (ns * R);
 #pragma unroll
         for (int e = RQ; e < 8; ++e) {
@@ -304,18 +340,19 @@ __device__ __forceinline__ void load_twiddles(const float2* table, int t, cf (&t
 // padded LDS buffers pa/pb (pass p writes pa for even p, pb for odd p).
 // Ends without a barrier: the last pass's reads of pa/pb may still be in
 // flight in other wavefronts.
-template <int N, int AS>
+template <int N, int AS, int SCHED = -1>
 __device__ __forceinline__ void stockham_inverse(cf (&v)[AS][8], float2* const (&pa)[AS], float2* const (&pb)[AS],
                                                  const cf (&tw)[Geo<N>::NPASS][8], int rbase) {
     using G = Geo<N>;
+    using P = FftPlan<N, SCHED>;
     constexpr int NP = G::NPASS;
     constexpr int TPA = G::TPA;
     const int t = threadIdx.x % TPA;
 #pragma unroll
     for (int p = 0; p < NP; ++p) {
-        const int R = (p < NP - 1) ? 8 : G::LAST_R;
+        const int R = P::radix(p);
         const int RQ = 8 / R;
-        const int ns = 1 << (3 * p);
+        const int ns = P::ns(p);
         if (p > 0) {
 #pragma unroll
             for (int u = 0; u < AS; ++u) {
@@ -453,7 +490,7 @@ struct ChainLds {
 // and each barrier covers AS A-scans; the per-position tables are shared).
 // WG: workgroup size, a multiple of N/8.  All lanes of the workgroup call
 // this with the same range.
-template <int N, int T, int MODE, int IW, int AS, int WG>
+template <int N, int T, int MODE, int IW, int AS, int WG, int SCHED = -1>
 __device__ __forceinline__ void chain_body(const ChainArgs A, float2* bufA, float2* bufB, float* xsep,
                                            typename ChainLds<N, MODE, IW, AS, WG>::DcSum* scan_part,
                                            float* mm_part, long long a_begin, long long a_end, long long a_step) {
@@ -481,6 +518,7 @@ __device__ __forceinline__ void chain_body(const ChainArgs A, float2* bufA, floa
 
     const int flags = A.flags;
     const bool ieee = A.in_type == IN_F32;
+    const bool shift = (flags & F_BITSHIFT) != 0;
 
     // ---- per-lane tables, read once per call
     // tap windows as LDS pointers into the lane's first staging buffer (the
@@ -502,7 +540,7 @@ __device__ __forceinline__ void chain_body(const ChainArgs A, float2* bufA, floa
         }
     }
     cf tw[NP][8];
-    load_twiddles<N>(A.twiddle, t, tw);
+    load_twiddles<N, SCHED>(A.twiddle, t, tw);
     cf fp[4];
     float bgv[4];
     if constexpr (!STATS) {
@@ -545,7 +583,7 @@ __device__ __forceinline__ void chain_body(const ChainArgs A, float2* bufA, floa
         float xv[AS][8];
 #pragma unroll
         for (int u = 0; u < AS; ++u) {
-            convert_raw<IW>(ieee, raw[u], xv[u]);
+            convert_raw<IW>(ieee, shift, raw[u], xv[u]);
             if constexpr (!DC) {
                 float4* dst = reinterpret_cast<float4*>(xs[u] + 8 * t);
                 dst[0] = make_float4(xv[u][0], xv[u][1], xv[u][2], xv[u][3]);
@@ -586,7 +624,7 @@ __device__ __forceinline__ void chain_body(const ChainArgs A, float2* bufA, floa
         }
 
         // ---- inverse FFT: NP Stockham passes
-        stockham_inverse<N, AS>(v, pa, pb, tw, rbase);
+        stockham_inverse<N, AS, SCHED>(v, pa, pb, tw, rbase);
         // after the last pass v[u][e] holds X[t + e*TPA]; e < 4 are the kept bins
 
 #pragma unroll
@@ -651,7 +689,14 @@ __device__ __forceinline__ void chain_body(const ChainArgs A, float2* bufA, floa
                     for (int o = 0; o < 4; ++o) res[o] = fmaf(A.ep_a, __fsqrt_rn(m2[o]), A.ep_b);
                 }
                 if (active) {
-                    OPE_GLOBAL float* dst = gmem(A.out) + a * G::HALF + t;
+                    long long row = a;
+                    if (flags & F_FLIP) {
+                        // cuda_kernels.cu:553-571: B-scans 0, 2, 4, ... reversed
+                        const long long b = a / A.ascans_per_bscan;
+                        if ((b & 1) == 0 && b + 1 < A.flip_bscans)
+                            row = b * A.ascans_per_bscan + (A.ascans_per_bscan - 1 - (a - b * A.ascans_per_bscan));
+                    }
+                    OPE_GLOBAL float* dst = gmem(A.out) + row * G::HALF + t;
 #pragma unroll
                     for (int o = 0; o < 4; ++o) {
                         const float r = (flags & F_BG) ? res[o] - bgv[o] : res[o];
@@ -681,7 +726,7 @@ __device__ __forceinline__ void chain_body(const ChainArgs A, float2* bufA, floa
 // One launch over a frame: persistent workgroups (the grid is sized to the
 // resident capacity) stride over the A-scans; in M_STATS mode each
 // workgroup takes one contiguous chunk so that segment runs are long.
-template <int N, int T, int MODE, int IW, int AS = 1>
+template <int N, int T, int MODE, int IW, int AS = 1, int SCHED = -1>
 __global__ void __launch_bounds__(Geo<N>::WG) __attribute__((amdgpu_waves_per_eu(OPE_WAVES_PER_EU)))
 chain_kernel(ChainArgs A) {
     constexpr int WG = Geo<N>::WG;
@@ -706,7 +751,7 @@ chain_kernel(ChainArgs A) {
         a_end = A.n_ascans;
         a_step = (long long)gridDim.x * SLOTS;
     }
-    chain_body<N, T, MODE, IW, AS, WG>(A, bufA, bufB, xsep, scan_part, mm_part, a_begin, a_end, a_step);
+    chain_body<N, T, MODE, IW, AS, WG, SCHED>(A, bufA, bufB, xsep, scan_part, mm_part, a_begin, a_end, a_step);
 }
 
 }  // namespace ope_hip

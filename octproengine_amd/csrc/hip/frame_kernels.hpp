@@ -80,4 +80,37 @@ __global__ void bg_subtract_kernel(float* __restrict__ out, long long total, int
     out[i] = out[i] - b;
 }
 
+// Sinusoidal scan correction (cuda_kernels.cu:408-431) from `in` to `out`
+// (total floats, rows of `width`): A-scan k of B-scan l becomes the linear
+// interpolation at scan position curve[k] between A-scans floor(curve[k])
+// and floor(curve[k]) + 1 of the flat buffer; the buffer's last A-scan is
+// copied.  Then, when use_bg is set, the post-process background
+// w*bg + off is subtracted (cuda_backend.cu:710-743 runs after this step).
+__global__ void sinusoidal_kernel(const float* __restrict__ in, float* __restrict__ out, long long total, int width,
+                                  int ascans, const float* __restrict__ curve, const float* __restrict__ bg, float bg_w,
+                                  float bg_off, int use_bg) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const long long row = i / width;
+    const int j = (int)(i - row * width);
+    float v;
+    if (i < total - width) {
+        const int k = (int)(row % ascans);
+        const long long l = row / ascans;
+        const float x = curve[k];
+        const int xi = (int)x;
+        // rows clamped to the buffer: only B-scans shorter than 3 A-scans or
+        // a partial last B-scan reach past it
+        const long long rows = total / width;
+        const long long r0 = min((long long)xi + l * ascans, rows - 1);
+        const long long r1 = min(r0 + 1, rows - 1);
+        const float f0 = in[r0 * width + j], f1 = in[r1 * width + j];
+        v = f0 + (f1 - f0) * (x - (float)xi);
+    } else {
+        v = in[i];
+    }
+    if (use_bg) v = v - __fadd_rn(__fmul_rn(bg_w, bg[j]), bg_off);
+    out[i] = v;
+}
+
 }  // namespace ope_hip

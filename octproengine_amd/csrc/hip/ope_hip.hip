@@ -90,6 +90,13 @@ struct ope_hip_ctx {
     // occupancy query is a runtime call; frames reuse one or two kernels)
     std::vector<std::pair<const void*, int>> occupancy;
 
+    // sinusoidal scan correction: positions per A-scan of a B-scan and the
+    // chain's output before the correction
+    float* d_sin_curve = nullptr;
+    int sin_curve_len = 0;
+    float* d_sin_tmp = nullptr;
+    size_t sin_tmp_cap = 0;  // floats
+
     // autofocus: window used by the differentiable chain, FP64 sums
     float* d_af_window = nullptr;
     double* d_af_acc = nullptr;
@@ -306,6 +313,22 @@ int validate_params(ope_hip_ctx* c, const ope_hip_params* p) {
 
 // ---------------------------------------------------------------- launches
 
+// Resident workgroups per CU of `kernel` (at least 1), queried once per
+// context and kernel.
+hipError_t cached_occupancy(ope_hip_ctx* c, const void* kernel, int wg, int* per_cu) {
+    for (const auto& e : c->occupancy)
+        if (e.first == kernel) {
+            *per_cu = e.second;
+            return hipSuccess;
+        }
+    int n = 0;
+    const hipError_t err = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kernel, wg, 0);
+    if (err != hipSuccess) return err;
+    *per_cu = n < 1 ? 1 : n;
+    c->occupancy.emplace_back(kernel, *per_cu);
+    return hipSuccess;
+}
+
 ChainLaunch pick_kernel(int n, int taps, int mode, int iw) {
     switch (n) {
         case 256: return pick_chain_n<256>(taps, mode, iw);
@@ -354,6 +377,8 @@ ChainArgs base_args(ope_hip_ctx* c) {
     a.log_h = (float)h;
     a.bg_w = p.bg_weight;
     a.bg_off = p.bg_offset;
+    if (p.bitshift) a.flags |= F_BITSHIFT;
+    a.ascans_per_bscan = p.ascans_per_bscan;
     a.acc = c->d_acc;
     return a;
 }
@@ -361,15 +386,8 @@ ChainArgs base_args(ope_hip_ctx* c) {
 int launch_chain(ope_hip_ctx* c, const ChainArgs& a, bool stats, long long work_ascans, hipStream_t s) {
     const int mode = (c->p.dc_removal_enabled ? M_DC : 0) | (stats ? M_STATS : 0);
     const ChainLaunch l = pick_kernel(c->n, c->taps, mode, (int)input_bytes(c->p.input_type));
-    const void* key = reinterpret_cast<const void*>(l.fn);
     int per_cu = 0;
-    for (const auto& e : c->occupancy)
-        if (e.first == key) per_cu = e.second;
-    if (per_cu == 0) {
-        HIP_TRY(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, key, l.wg, 0));
-        if (per_cu < 1) per_cu = 1;
-        c->occupancy.emplace_back(key, per_cu);
-    }
+    HIP_TRY(c, cached_occupancy(c, reinterpret_cast<const void*>(l.fn), l.wg, &per_cu));
     long long groups = (work_ascans + l.slots - 1) / l.slots;
     long long grid = std::min<long long>(groups, (long long)per_cu * c->cu_count);
     if (grid < 1) return OPE_HIP_OK;
@@ -381,6 +399,50 @@ int launch_chain(ope_hip_ctx* c, const ChainArgs& a, bool stats, long long work_
 int zero_acc(ope_hip_ctx* c, hipStream_t s) {
     const long long n = (long long)kFpnSegments * (c->n / 2) * 3;
     hipLaunchKernelGGL(fill_zero_f64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, c->d_acc, n);
+    HIP_TRY(c, hipGetLastError());
+    return OPE_HIP_OK;
+}
+
+// Positions of the sinusoidal scan correction for the context's B-scan
+// length (fillSinusoidalScanCorrectionCurve, cuda_kernels.cu:433-437:
+// acos in float, the product in double).
+int ensure_sin_curve(ope_hip_ctx* c) {
+    const int a = c->p.ascans_per_bscan;
+    if (c->d_sin_curve && c->sin_curve_len == a) return OPE_HIP_OK;
+    std::vector<float> curve(a);
+    for (int k = 0; k < a; ++k) {
+        const float arg = (float)(1.0 - (2.0 * (double)(float)k) / (double)(float)a);
+        curve[k] = (float)(((double)(float)a / M_PI) * (double)std::acos(arg));
+    }
+    HIP_TRY(c, hipSetDevice(c->device));
+    if (c->d_sin_curve) HIP_TRY(c, hipFree(c->d_sin_curve));
+    c->d_sin_curve = nullptr;
+    HIP_TRY(c, hipMalloc(&c->d_sin_curve, a * sizeof(float)));
+    HIP_TRY(c, hipMemcpy(c->d_sin_curve, curve.data(), a * sizeof(float), hipMemcpyHostToDevice));
+    c->sin_curve_len = a;
+    return OPE_HIP_OK;
+}
+
+// The chain's output before the sinusoidal correction: n_ascans rows.
+int ensure_sin_tmp(ope_hip_ctx* c, long long n_ascans) {
+    const size_t need = (size_t)n_ascans * (c->n / 2);
+    if (need <= c->sin_tmp_cap) return OPE_HIP_OK;
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, hipDeviceSynchronize());  // frames on any stream may still read it
+    if (c->d_sin_tmp) HIP_TRY(c, hipFree(c->d_sin_tmp));
+    c->d_sin_tmp = nullptr;
+    c->sin_tmp_cap = 0;
+    HIP_TRY(c, hipMalloc(&c->d_sin_tmp, need * sizeof(float)));
+    c->sin_tmp_cap = need;
+    return OPE_HIP_OK;
+}
+
+int launch_sinusoidal(ope_hip_ctx* c, const float* in, float* out, long long n_ascans, bool use_bg, hipStream_t s) {
+    const int half = c->n / 2;
+    const long long total = n_ascans * half;
+    hipLaunchKernelGGL(sinusoidal_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, in, out, total, half,
+                       c->p.ascans_per_bscan, c->d_sin_curve, c->d_bg, c->p.bg_weight, c->p.bg_offset,
+                       use_bg ? 1 : 0);
     HIP_TRY(c, hipGetLastError());
     return OPE_HIP_OK;
 }
@@ -435,11 +497,29 @@ int run_frame(ope_hip_ctx* c, const void* d_in, float* d_out, long long n_ascans
         c->fpn_requested = false;
     }
     if (p.fpn_removal && c->have_fpn) a.flags |= F_FPN;
+    if (p.bscan_flip) {
+        a.flags |= F_FLIP;
+        a.flip_bscans = n_ascans / p.ascans_per_bscan;
+    }
 
+    // CUDA order after the grayscale map: B-scan flip (in the chain's
+    // epilogue), sinusoidal correction, background (cuda_backend.cu:689-743)
+    const bool sinus = p.sinusoidal_correction != 0;
     const bool record_bg = p.bg_removal && c->bg_requested;
-    if (p.bg_removal && c->have_bg && !record_bg) a.flags |= F_BG;
+    const bool subtract_bg = p.bg_removal && c->have_bg && !record_bg;
+    if (subtract_bg && !sinus) a.flags |= F_BG;
+    if (sinus) {
+        int rc = ensure_sin_curve(c);
+        if (!rc) rc = ensure_sin_tmp(c, n_ascans);
+        if (rc) return rc;
+        a.out = c->d_sin_tmp;
+    }
     int rc = launch_chain(c, a, false, n_ascans, s);
     if (rc) return rc;
+    if (sinus) {
+        rc = launch_sinusoidal(c, c->d_sin_tmp, d_out, n_ascans, subtract_bg, s);
+        if (rc) return rc;
+    }
     if (record_bg) {
         hipLaunchKernelGGL(bg_record_kernel, dim3((half + 255) / 256), dim3(256), 0, s, d_out, n_ascans, half,
                            c->d_bg);
@@ -469,7 +549,7 @@ void free_ctx(ope_hip_ctx* c) {
     }
     for (void* p : {(void*)c->d_tap_base, (void*)c->d_tap_w, (void*)c->d_phasor, (void*)c->d_twiddle,
                     (void*)c->d_fpn, (void*)c->d_bg, (void*)c->d_acc, (void*)c->d_af_window,
-                    (void*)c->d_af_acc})
+                    (void*)c->d_af_acc, (void*)c->d_sin_curve, (void*)c->d_sin_tmp})
         if (p) (void)hipFree(p);
     for (hipStream_t s : {c->s_in, c->s_comp, c->s_out})
         if (s) (void)hipStreamDestroy(s);
@@ -531,6 +611,7 @@ int autofocus_pass(ope_hip_ctx* c, const void* d_in, long long n_ascans, const d
     a.window = c->d_af_window;
     a.twiddle = c->d_twiddle;
     a.in_type = input_class(p.input_type);
+    a.bitshift = p.bitshift;
     a.dc_window = p.dc_window_size;
     a.d2 = (float)theta[0];
     a.d3 = (float)theta[1];
@@ -543,8 +624,7 @@ int autofocus_pass(ope_hip_ctx* c, const void* d_in, long long n_ascans, const d
     a.acc = c->d_af_acc;
     const AutofocusLaunch l = pick_autofocus(n, c->taps, iw, p.dc_removal_enabled != 0, bwd);
     int per_cu = 0;
-    HIP_TRY(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(l.fn), l.wg, 0));
-    if (per_cu < 1) per_cu = 1;
+    HIP_TRY(c, cached_occupancy(c, reinterpret_cast<const void*>(l.fn), l.wg, &per_cu));
     const long long groups = (n_ascans + l.slots - 1) / l.slots;
     const long long grid = std::min<long long>(groups, (long long)per_cu * c->cu_count);
     hipLaunchKernelGGL(l.fn, dim3((unsigned)grid), dim3(l.wg), 0, s, a);
@@ -603,6 +683,9 @@ int build_ragged(ope_hip_ragged* r, ope_hip_ctx* const* ctxs, int n_ctx, const o
         if (c->bg_requested && c->p.bg_removal)
             return ragged_fail(r, OPE_HIP_ESTATE, "context of length %d has a background recording pending",
                                c->n);
+        if (c->p.bscan_flip || c->p.sinusoidal_correction)
+            return ragged_fail(r, OPE_HIP_EUNSUP,
+                               "B-scan flip and sinusoidal correction act on frames, not on ragged requests");
     }
     const ope_hip_ctx* c0 = ctxs[0];
     r->device = c0->device;
@@ -1116,6 +1199,15 @@ int ope_hip_graph_create(ope_hip_ctx* c, const void* const* d_in, float* const* 
             reinterpret_cast<uintptr_t>(d_out[f]) % 4) {
             g_create_err = "frame buffers must be non-null, d_in 16-byte and d_out 4-byte aligned";
             return OPE_HIP_EINVAL;
+        }
+    }
+    if (p.sinusoidal_correction) {
+        // allocations cannot happen inside the capture
+        int rc0 = ensure_sin_curve(c);
+        if (!rc0) rc0 = ensure_sin_tmp(c, ascans_per_frame);
+        if (rc0) {
+            g_create_err = c->err;
+            return rc0;
         }
     }
     ope_hip_graph* g = new (std::nothrow) ope_hip_graph();

@@ -45,16 +45,23 @@ FPN_SEGMENTS = 8  # src/backends/cpu/cpu_backend.cpp:306
 # Per-A-scan stages (src/backends/cpu/cpu_kernels.tpp)
 # ---------------------------------------------------------------------------
 
-def convert_input(raw: np.ndarray, ieee_float32: bool = True) -> np.ndarray:
+def convert_input(raw: np.ndarray, ieee_float32: bool = True, bitshift: bool = False) -> np.ndarray:
     """Raw samples -> float64 (cpu_kernels.tpp:24-54).
 
     The reference reads by bit depth (<=8 -> uint8, <=16 -> uint16, else
     uint32).  For FLOAT32 input the reference reinterprets the bits as uint32
     (cpu_kernels.tpp:47-52 with types.cpp:15); this build reads IEEE floats
     (``ieee_float32=True``), a documented divergence.
+    ``bitshift`` is the CUDA backend's input option
+    (cuda_kernels.cu:45-70): 8/16-bit samples >> 4, 32-bit integers / 2^32;
+    it leaves IEEE float input unchanged.
     """
     if raw.dtype == np.float32 and not ieee_float32:
         raw = raw.view(np.uint32)
+    if bitshift and raw.dtype != np.float32:
+        if raw.dtype.itemsize <= 2:
+            return (raw.astype(np.uint32) >> 4).astype(np.float64)
+        return raw.astype(np.float64) / 4294967296.0
     return raw.astype(np.float64)
 
 
@@ -292,6 +299,9 @@ class ChainParams:
     bg_removal: bool = False
     bg_weight: float = 1.0
     bg_offset: float = 0.0
+    bitshift: bool = False
+    bscan_flip: bool = False
+    sinusoidal: bool = False
 
 
 @dataclass
@@ -310,7 +320,7 @@ class ChainState:
 def spectra(raw: np.ndarray, p: ChainParams, curve, window, phasor, ieee_float32=True) -> np.ndarray:
     """Complex IFFT output per A-scan, before FPN (cpu_backend.cpp:184-243)."""
     n = p.signal_length
-    x = convert_input(raw, ieee_float32).reshape(-1, n)
+    x = convert_input(raw, ieee_float32, p.bitshift).reshape(-1, n)
     if p.dc_removal:
         x = rolling_average_dc(x, p.dc_window)
     if p.resampling:
@@ -370,6 +380,10 @@ def process_frame(raw, p: ChainParams, curve, window, phasor, state: ChainState 
                                      p.addend, p.grayscale_min == p.grayscale_max)
     else:
         out = linear_scale_and_truncate(X, p.multiplicator, p.grayscale_min, p.grayscale_max, p.addend)
+    if p.bscan_flip:
+        out = bscan_flip(out, p.ascans_per_bscan)
+    if p.sinusoidal:
+        out = sinusoidal_correction(out, p.ascans_per_bscan, sinusoidal_curve(p.ascans_per_bscan))
     if p.bg_removal:
         if state.bg_requested:
             state.bg_profile = record_background(out)
@@ -377,6 +391,52 @@ def process_frame(raw, p: ChainParams, curve, window, phasor, state: ChainState 
         if state.bg_profile is not None and len(state.bg_profile):
             out = subtract_background(out, state.bg_profile, p.bg_weight, p.bg_offset)
     return out
+
+
+# ---------------------------------------------------------------------------
+# Steps of the CUDA backend only (the CPU backend ignores these flags)
+# ---------------------------------------------------------------------------
+
+def bscan_flip(out: np.ndarray, ascans_per_bscan: int) -> np.ndarray:
+    """cuda_bscanFlip (cuda_kernels.cu:553-571) as launched by
+    cuda_backend.cu:689-694: the A-scan order of B-scans 0, 2, 4, ... is
+    reversed.  The launch covers the first half of the frame's samples, so
+    with an odd B-scan count the last (even-indexed) B-scan is left as is."""
+    out = np.array(out, copy=True)
+    a = ascans_per_bscan
+    b_count = out.shape[0] // a
+    for b in range(0, b_count, 2):
+        if b + 1 < b_count:
+            out[b * a:(b + 1) * a] = out[b * a:(b + 1) * a][::-1]
+    return out
+
+
+def sinusoidal_curve(ascans_per_bscan: int) -> np.ndarray:
+    """fillSinusoidalScanCorrectionCurve (cuda_kernels.cu:433-437):
+    (A / pi) * acos(1 - 2k/A), acos in float, the product in double."""
+    a = ascans_per_bscan
+    k = np.arange(a, dtype=np.float64)
+    arg = (1.0 - (2.0 * k.astype(np.float32).astype(np.float64)) / float(np.float32(a))).astype(np.float32)
+    return ((float(np.float32(a)) / np.pi) * np.arccos(arg).astype(np.float64)).astype(np.float32)
+
+
+def sinusoidal_correction(out: np.ndarray, ascans_per_bscan: int, curve: np.ndarray) -> np.ndarray:
+    """sinusoidalScanCorrection (cuda_kernels.cu:408-431): A-scan k of B-scan
+    l becomes the linear interpolation at scan position curve[k] between
+    A-scans floor(curve[k]) and floor(curve[k]) + 1 (flat buffer indexing);
+    the buffer's last A-scan is not rewritten."""
+    flat = np.asarray(out, dtype=np.float64)
+    rows, width = flat.shape
+    a = ascans_per_bscan
+    res = flat.copy()
+    for i in range(rows - 1):
+        k, l = i % a, i // a
+        x = float(curve[k])
+        xi = int(x)
+        r0 = min(xi + l * a, rows - 1)
+        r1 = min(r0 + 1, rows - 1)
+        res[i] = flat[r0] + (flat[r1] - flat[r0]) * (x - xi)
+    return res
 
 
 # ---------------------------------------------------------------------------

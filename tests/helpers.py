@@ -42,7 +42,8 @@ def chain_params(p: O.ChainParams, dtype=np.uint16):
         grayscale_max=p.grayscale_max, multiplicator=p.multiplicator, addend=p.addend,
         fpn_removal=p.fpn_removal, fpn_bscan_count=p.fpn_bscan_count,
         fpn_continuous=p.fpn_continuous, bg_removal=p.bg_removal, bg_weight=p.bg_weight,
-        bg_offset=p.bg_offset)
+        bg_offset=p.bg_offset, bitshift=p.bitshift, bscan_flip=p.bscan_flip,
+        sinusoidal_correction=p.sinusoidal)
 
 
 def to_device(raw: np.ndarray):

@@ -255,3 +255,46 @@ def test_cpu32_restatement_matches_fp64_oracle(method):
     ref = O.process_frame(raw, p, t.curve, t.window, t.phasor)
     got = cpu32(p, raw, t)
     assert db_error(got, ref) <= 1e-3
+
+
+# ---------------------------------------------------------------- CUDA-only steps
+
+
+def test_bscan_flip_restatement():
+    a, width = 5, 3
+    for b in (1, 2, 3, 4):
+        x = np.arange(a * b * width, dtype=np.float64).reshape(a * b, width)
+        y = O.bscan_flip(x, a)
+        for k in range(b):
+            blk, src = y[k * a:(k + 1) * a], x[k * a:(k + 1) * a]
+            flipped = k % 2 == 0 and k + 1 < b  # the launch covers half of the samples
+            assert np.array_equal(blk, src[::-1] if flipped else src), (b, k)
+        assert np.array_equal(O.bscan_flip(y, a), x)  # an involution
+    # planted mistake: flipping the odd B-scans instead must fail the check
+    x = np.arange(8 * 2, dtype=np.float64).reshape(8, 2)
+    assert not np.array_equal(O.bscan_flip(x, 4)[4:], x[4:][::-1])
+
+
+def test_sinusoidal_curve_closed_form():
+    for a in (4, 64, 1024):
+        c = O.sinusoidal_curve(a).astype(np.float64)
+        k = np.arange(a)
+        assert c[0] == 0.0
+        assert np.all(np.diff(c) > 0)
+        assert np.allclose(c, a / np.pi * np.arccos(1 - 2 * k / a), rtol=1e-6, atol=1e-4)
+        if a % 2 == 0:
+            assert abs(c[a // 2] - a / 2) < 1e-4 * a
+
+
+def test_sinusoidal_correction_reproduces_linear_fields():
+    """Linear interpolation along the scan is exact on fields linear in the
+    flat A-scan index: row i of B-scan l becomes curve[k] + l*A."""
+    a, b, width = 32, 3, 4
+    curve = O.sinusoidal_curve(a)
+    rows = np.arange(a * b, dtype=np.float64)
+    field = np.repeat((2.5 * rows + 1.0)[:, None], width, axis=1)
+    out = O.sinusoidal_correction(field, a, curve)
+    k, l = rows[:-1] % a, rows[:-1] // a
+    want = 2.5 * (curve[k.astype(int)].astype(np.float64) + l * a) + 1.0
+    assert np.allclose(out[:-1, 0], want, rtol=0, atol=1e-9)
+    assert np.array_equal(out[-1], field[-1])  # the last A-scan is not rewritten
