@@ -207,6 +207,28 @@ OPE_HIP_API int ope_hip_host_free(void* ptr);
 OPE_HIP_API int ope_hip_op_min_variance_mean(ope_hip_ctx* ctx, const float* input, int width,
                                              int height, int segments, float* out);
 
+/* ---- Launch autotuning.  No counterpart in the reference (its CUDA
+ * backend launches fixed grids, cuda_backend.cu:486-743).  Times the chain's
+ * launch variants for the context's current configuration (interpolation,
+ * DC removal, input width): 1 or 2 A-scans per lane, the position of the
+ * FFT's small-radix pass, 1 .. resident workgroups per CU; each variant
+ * runs once untimed and 3 times timed on d_in -> d_out (n_ascans A-scans;
+ * d_out is overwritten).  The fastest is used by later frames of that
+ * configuration on this context.  With d_in and d_out NULL the context's own
+ * first frame slot is used.  choice (may be NULL) receives {A-scans per
+ * lane, small-radix pass position, workgroups per CU}; best_us (may be
+ * NULL) the fastest variant's time per launch.  Synchronous. */
+OPE_HIP_API int ope_hip_autotune(ope_hip_ctx* ctx, const void* d_in, float* d_out, int64_t n_ascans,
+                                 void* stream, int* choice, float* best_us);
+
+/* Pins the launch variant of the context's current configuration instead
+ * of tuning it: A-scans per lane (1 or 2), small-radix pass position
+ * (0 .. passes-1; the last pass is the default; other positions exist for
+ * one A-scan per lane only: OPE_HIP_EUNSUP), workgroups per CU (0: all
+ * that are resident). */
+OPE_HIP_API int ope_hip_set_launch_variant(ope_hip_ctx* ctx, int ascans_per_lane, int small_radix_pass,
+                                           int workgroups_per_cu);
+
 /* ---- Graph-captured frames.  No counterpart in the reference (one
  * process() call per frame, cuda_backend.cu:442-773).  For callers that
  * replay a fixed ring of device frames (volume acquisition into

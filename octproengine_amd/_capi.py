@@ -53,6 +53,8 @@ SIGNATURES = {
     "ope_hip_autofocus_loss": (c_int, [c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_void_p]),
     "ope_hip_autofocus_grad": (c_int, [c_void_p, c_void_p, c_int64, c_void_p, ctypes.c_double,
                                        ctypes.c_double, c_void_p, c_void_p]),
+    "ope_hip_autotune": (c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_void_p]),
+    "ope_hip_set_launch_variant": (c_int, [c_void_p, c_int, c_int, c_int]),
     "ope_hip_graph_create": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int64, POINTER(c_void_p)]),
     "ope_hip_graph_launch": (c_int, [c_void_p, c_void_p]),
     "ope_hip_graph_last_error": (c_char_p, [c_void_p]),
@@ -248,6 +250,20 @@ class Context:
 
     def synchronize(self) -> None:
         self._check(lib.ope_hip_synchronize(self._h))
+
+    def autotune(self, d_in: int = 0, d_out: int = 0, n_ascans: int = 0, stream: int = 0):
+        """Times the chain's launch variants on d_in -> d_out (the context's
+        own frame slot when 0) and keeps the fastest for this configuration.
+        Returns ({as, sched, per_cu}, microseconds per launch)."""
+        choice = np.zeros(3, dtype=np.int32)
+        best = ctypes.c_float(0.0)
+        self._check(lib.ope_hip_autotune(self._h, c_void_p(d_in or None), c_void_p(d_out or None), int(n_ascans),
+                                         c_void_p(stream or None), choice.ctypes.data, ctypes.byref(best)))
+        return {"ascans_per_lane": int(choice[0]), "small_radix_pass": int(choice[1]),
+                "workgroups_per_cu": int(choice[2])}, float(best.value)
+
+    def set_launch_variant(self, ascans_per_lane: int, small_radix_pass: int, workgroups_per_cu: int = 0) -> None:
+        self._check(lib.ope_hip_set_launch_variant(self._h, ascans_per_lane, small_radix_pass, workgroups_per_cu))
 
     # -- autofocus (theta = d2, d3, c0, c1, c2, c3)
     def autofocus_loss(self, d_in: int, n_ascans: int, theta, stream: int = 0):

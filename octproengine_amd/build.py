@@ -32,6 +32,9 @@ HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-munsafe-fp-atomics", "-Wall", "-Wno-unused-function",
+    # occupancy hints the LDS footprint overrides (Lanczos with two A-scans per
+    # lane at N <= 512) are reported per kernel; they are expected
+    "-Wno-pass-failed",
 ]
 CHAIN_LENGTHS = (256, 512, 1024, 2048, 4096)
 RAGGED_TAPS = (1, 2, 4, 16)
@@ -76,10 +79,11 @@ def build_hip(verbose_resources: bool = False) -> Path:
     cmds.append([HIPCC, *HIP_FLAGS, *extra, f"-I{INCLUDE}", "-c", str(hip_dir / "ope_hip.hip"), "-o", str(host_obj)])
     objs.append(host_obj)
     for n in CHAIN_LENGTHS:
-        obj = OBJ_DIR / f"chain_n{n}.o"
-        cmds.append([HIPCC, *HIP_FLAGS, *extra, f"-I{INCLUDE}", f"-DOPE_CHAIN_N={n}", "-c",
-                     str(hip_dir / "chain_instances.hip"), "-o", str(obj)])
-        objs.append(obj)
+        for a_s in (1, 2):
+            obj = OBJ_DIR / f"chain_n{n}_as{a_s}.o"
+            cmds.append([HIPCC, *HIP_FLAGS, *extra, f"-I{INCLUDE}", f"-DOPE_CHAIN_N={n}", f"-DOPE_CHAIN_AS={a_s}",
+                         "-c", str(hip_dir / "chain_instances.hip"), "-o", str(obj)])
+            objs.append(obj)
     for t in RAGGED_TAPS:
         obj = OBJ_DIR / f"ragged_t{t}.o"
         cmds.append([HIPCC, *HIP_FLAGS, *extra, f"-I{INCLUDE}", f"-DOPE_RAGGED_T={t}", "-c",

@@ -116,6 +116,10 @@ void HipBackend::initialize(const ProcessorConfiguration& config) {
     if (pendingFpnRequest_) check(ope_hip_request_fpn(ctx_), "requestFixedPatternNoiseDetermination");
     if (pendingBgRequest_) check(ope_hip_request_bg_recording(ctx_), "requestPostProcessBackgroundRecording");
     pendingFpnRequest_ = pendingBgRequest_ = false;
+    // fastest launch variant for this configuration, timed on the context's
+    // own frame slot (a few milliseconds); the default variant stays if this
+    // fails
+    (void)ope_hip_autotune(ctx_, nullptr, nullptr, 0, nullptr, nullptr, nullptr);
 }
 
 void HipBackend::cleanup() {

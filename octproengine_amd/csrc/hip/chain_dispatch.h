@@ -18,9 +18,13 @@ struct ChainLaunch {
 };
 
 // taps: 1, 2, 4 or 16; mode: M_DC / M_STATS bits; iw: input sample width in
-// bytes (1, 2, 4).  Explicitly instantiated for N = 256 ... 4096.
-template <int N>
-ChainLaunch pick_chain_n(int taps, int mode, int iw);
+// bytes (1, 2, 4); AS: A-scans per lane (1, or 2 for the modes without
+// statistics); sched: position of the small-radix FFT pass (0 ..
+// passes - 1; other values: the last pass).  fn is null for a combination
+// that is not instantiated.  Explicitly instantiated for N = 256 ... 4096
+// and AS = 1, 2 (chain_instances.hip).
+template <int N, int AS>
+ChainLaunch pick_chain(int taps, int mode, int iw, int sched);
 
 using RaggedFn = void (*)(RaggedArgs);
 

@@ -1,56 +1,75 @@
-// Chain-kernel instantiations of one signal length, OPE_CHAIN_N (set on the
-// compiler line).  The table covers taps {1, 2, 4, 16} x modes {plain, DC,
-// statistics, DC + statistics} x input widths {1, 2, 4} bytes.
+// Chain-kernel instantiations of one signal length and one number of
+// A-scans per lane, OPE_CHAIN_N and OPE_CHAIN_AS (set on the compiler line).
+//  AS = 1: taps {1, 2, 4, 16} x modes {plain, DC} x input widths {1, 2, 4}
+//          bytes x every radix placement, plus the FPN-statistics modes with
+//          the default placement;
+//  AS = 2: taps x modes {plain, DC} x widths, default placement.
 #include "chain_dispatch.h"
 
-#ifndef OPE_CHAIN_N
-#error "compile with -DOPE_CHAIN_N=<signal length>"
+#if !defined(OPE_CHAIN_N) || !defined(OPE_CHAIN_AS)
+#error "compile with -DOPE_CHAIN_N=<signal length> -DOPE_CHAIN_AS=<1|2>"
 #endif
 
 namespace ope_hip {
 namespace {
 
-// A-scans per lane.  Two (instruction-level parallelism) measured slower
-// than one on gfx950 for every mode: the extra VGPRs and LDS cost a wave of
-// occupancy per SIMD, which hides more latency than the second A-scan does.
-constexpr int kAscansPerLane = 1;
-
-template <int N, int MODE, int IW>
+template <int N, int MODE, int IW, int AS, int SCHED>
 KernelFn pick_taps(int taps) {
-    constexpr int AS = kAscansPerLane;
     switch (taps) {
-        case 1: return chain_kernel<N, 1, MODE, IW, AS>;
-        case 2: return chain_kernel<N, 2, MODE, IW, AS>;
-        case 4: return chain_kernel<N, 4, MODE, IW, AS>;
-        default: return chain_kernel<N, 16, MODE, IW, AS>;
+        case 1: return chain_kernel<N, 1, MODE, IW, AS, SCHED>;
+        case 2: return chain_kernel<N, 2, MODE, IW, AS, SCHED>;
+        case 4: return chain_kernel<N, 4, MODE, IW, AS, SCHED>;
+        default: return chain_kernel<N, 16, MODE, IW, AS, SCHED>;
     }
 }
 
-template <int N, int MODE>
+template <int N, int MODE, int AS, int SCHED>
 KernelFn pick_width(int taps, int iw) {
     switch (iw) {
-        case 1: return pick_taps<N, MODE, 1>(taps);
-        case 2: return pick_taps<N, MODE, 2>(taps);
-        default: return pick_taps<N, MODE, 4>(taps);
+        case 1: return pick_taps<N, MODE, 1, AS, SCHED>(taps);
+        case 2: return pick_taps<N, MODE, 2, AS, SCHED>(taps);
+        default: return pick_taps<N, MODE, 4, AS, SCHED>(taps);
     }
+}
+
+// the placements of the small-radix pass that differ for this N
+template <int N, int MODE, int AS>
+KernelFn pick_sched(int taps, int iw, int sched) {
+    constexpr int NP = FftPlan<N, -1>::NP;
+    constexpr int LAST = NP - 1;
+    if constexpr (AS == 1 && FftPlan<N, -1>::SMALL != 8) {
+        switch (sched) {
+            case 0: return pick_width<N, MODE, AS, 0>(taps, iw);
+            case 1: return pick_width<N, MODE, AS, (1 < LAST ? 1 : LAST)>(taps, iw);
+            case 2: return pick_width<N, MODE, AS, (2 < LAST ? 2 : LAST)>(taps, iw);
+            default: break;
+        }
+    }
+    return pick_width<N, MODE, AS, LAST>(taps, iw);
 }
 
 }  // namespace
 
-template <int N>
-ChainLaunch pick_chain_n(int taps, int mode, int iw) {
+template <int N, int AS>
+ChainLaunch pick_chain(int taps, int mode, int iw, int sched) {
     ChainLaunch l;
     l.wg = Geo<N>::WG;
-    l.slots = Geo<N>::APW * kAscansPerLane;
+    l.slots = Geo<N>::APW * AS;
+    constexpr int LAST = FftPlan<N, -1>::NP - 1;
+    l.fn = nullptr;
     switch (mode) {
-        case 0: l.fn = pick_width<N, 0>(taps, iw); break;
-        case M_DC: l.fn = pick_width<N, M_DC>(taps, iw); break;
-        case M_STATS: l.fn = pick_width<N, M_STATS>(taps, iw); break;
-        default: l.fn = pick_width<N, M_DC | M_STATS>(taps, iw); break;
+        case 0: l.fn = pick_sched<N, 0, AS>(taps, iw, sched); break;
+        case M_DC: l.fn = pick_sched<N, M_DC, AS>(taps, iw, sched); break;
+        case M_STATS:
+            if constexpr (AS == 1) l.fn = pick_width<N, M_STATS, 1, LAST>(taps, iw);
+            break;
+        default:
+            if constexpr (AS == 1) l.fn = pick_width<N, M_DC | M_STATS, 1, LAST>(taps, iw);
+            break;
     }
     return l;
 }
 
-template ChainLaunch pick_chain_n<OPE_CHAIN_N>(int, int, int);
+template ChainLaunch pick_chain<OPE_CHAIN_N, OPE_CHAIN_AS>(int, int, int, int);
 
 }  // namespace ope_hip

@@ -726,8 +726,11 @@ __device__ __forceinline__ void chain_body(const ChainArgs A, float2* bufA, floa
 // One launch over a frame: persistent workgroups (the grid is sized to the
 // resident capacity) stride over the A-scans; in M_STATS mode each
 // workgroup takes one contiguous chunk so that segment runs are long.
+// Lanczos (T = 16) holds 128 tap weights per lane: left to itself the
+// compiler spends AGPRs on the DC prologue and drops to one wave per SIMD;
+// two waves per SIMD with a few spills measured faster.
 template <int N, int T, int MODE, int IW, int AS = 1, int SCHED = -1>
-__global__ void __launch_bounds__(Geo<N>::WG) __attribute__((amdgpu_waves_per_eu(OPE_WAVES_PER_EU)))
+__global__ void __launch_bounds__(Geo<N>::WG) __attribute__((amdgpu_waves_per_eu(T == 16 ? 2 : OPE_WAVES_PER_EU)))
 chain_kernel(ChainArgs A) {
     constexpr int WG = Geo<N>::WG;
     using L = ChainLds<N, MODE, IW, AS, WG>;

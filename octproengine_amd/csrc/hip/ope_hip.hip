@@ -90,6 +90,13 @@ struct ope_hip_ctx {
     // occupancy query is a runtime call; frames reuse one or two kernels)
     std::vector<std::pair<const void*, int>> occupancy;
 
+    // launch variant chosen by ope_hip_autotune for a kernel configuration
+    struct Tuning {
+        int taps, mode, iw;      // configuration
+        int as, sched, per_cu;   // A-scans per lane, small-radix pass, workgroups per CU
+    };
+    std::vector<Tuning> tuning;
+
     // sinusoidal scan correction: positions per A-scan of a B-scan and the
     // chain's output before the correction
     float* d_sin_curve = nullptr;
@@ -329,15 +336,22 @@ hipError_t cached_occupancy(ope_hip_ctx* c, const void* kernel, int wg, int* per
     return hipSuccess;
 }
 
-ChainLaunch pick_kernel(int n, int taps, int mode, int iw) {
+template <int AS>
+ChainLaunch pick_kernel_as(int n, int taps, int mode, int iw, int sched) {
     switch (n) {
-        case 256: return pick_chain_n<256>(taps, mode, iw);
-        case 512: return pick_chain_n<512>(taps, mode, iw);
-        case 1024: return pick_chain_n<1024>(taps, mode, iw);
-        case 2048: return pick_chain_n<2048>(taps, mode, iw);
-        default: return pick_chain_n<4096>(taps, mode, iw);
+        case 256: return pick_chain<256, AS>(taps, mode, iw, sched);
+        case 512: return pick_chain<512, AS>(taps, mode, iw, sched);
+        case 1024: return pick_chain<1024, AS>(taps, mode, iw, sched);
+        case 2048: return pick_chain<2048, AS>(taps, mode, iw, sched);
+        default: return pick_chain<4096, AS>(taps, mode, iw, sched);
     }
 }
+
+ChainLaunch pick_kernel(int n, int taps, int mode, int iw, int as = 1, int sched = -1) {
+    return as == 2 ? pick_kernel_as<2>(n, taps, mode, iw, sched) : pick_kernel_as<1>(n, taps, mode, iw, sched);
+}
+
+int fft_passes(int n) { return (ilog2(n) + 2) / 3; }
 
 ChainArgs base_args(ope_hip_ctx* c) {
     ChainArgs a{};
@@ -383,17 +397,31 @@ ChainArgs base_args(ope_hip_ctx* c) {
     return a;
 }
 
-int launch_chain(ope_hip_ctx* c, const ChainArgs& a, bool stats, long long work_ascans, hipStream_t s) {
-    const int mode = (c->p.dc_removal_enabled ? M_DC : 0) | (stats ? M_STATS : 0);
-    const ChainLaunch l = pick_kernel(c->n, c->taps, mode, (int)input_bytes(c->p.input_type));
+// Launches one chain variant: `as` A-scans per lane, small-radix pass at
+// `sched`, `per_cu` workgroups per CU (0: as many as are resident).
+int launch_variant(ope_hip_ctx* c, const ChainArgs& a, int mode, int as, int sched, int per_cu_want,
+                   long long work_ascans, hipStream_t s) {
+    const ChainLaunch l = pick_kernel(c->n, c->taps, mode, (int)input_bytes(c->p.input_type), as, sched);
+    if (!l.fn) return fail(c, OPE_HIP_EUNSUP, "chain variant not built (mode %d, %d A-scans per lane)", mode, as);
     int per_cu = 0;
     HIP_TRY(c, cached_occupancy(c, reinterpret_cast<const void*>(l.fn), l.wg, &per_cu));
+    if (per_cu_want > 0 && per_cu_want < per_cu) per_cu = per_cu_want;
     long long groups = (work_ascans + l.slots - 1) / l.slots;
     long long grid = std::min<long long>(groups, (long long)per_cu * c->cu_count);
     if (grid < 1) return OPE_HIP_OK;
     hipLaunchKernelGGL(l.fn, dim3((unsigned)grid), dim3(l.wg), 0, s, a);
     HIP_TRY(c, hipGetLastError());
     return OPE_HIP_OK;
+}
+
+int launch_chain(ope_hip_ctx* c, const ChainArgs& a, bool stats, long long work_ascans, hipStream_t s) {
+    const int mode = (c->p.dc_removal_enabled ? M_DC : 0) | (stats ? M_STATS : 0);
+    const int iw = (int)input_bytes(c->p.input_type);
+    if (!stats)
+        for (const auto& t : c->tuning)
+            if (t.taps == c->taps && t.mode == mode && t.iw == iw)
+                return launch_variant(c, a, mode, t.as, t.sched, t.per_cu, work_ascans, s);
+    return launch_variant(c, a, mode, 1, -1, 0, work_ascans, s);
 }
 
 int zero_acc(ope_hip_ctx* c, hipStream_t s) {
@@ -870,6 +898,7 @@ int ope_hip_create(int device, const ope_hip_params* params, int n_slots, ope_hi
     for (auto& s : c->slots) {
         CREATE_TRY(hipMalloc(&s.d_in, frame_ascans * n * input_bytes(params->input_type)));
         CREATE_TRY(hipMalloc(&s.d_out, frame_ascans * half * sizeof(float)));
+        CREATE_TRY(hipMemset(s.d_in, 0, frame_ascans * n * input_bytes(params->input_type)));
         CREATE_TRY(hipEventCreateWithFlags(&s.h2d_done, hipEventDisableTiming));
         CREATE_TRY(hipEventCreateWithFlags(&s.comp_done, hipEventDisableTiming));
         CREATE_TRY(hipEventCreateWithFlags(&s.d2h_done, hipEventDisableTiming));
@@ -1271,6 +1300,122 @@ int ope_hip_graph_destroy(ope_hip_graph* g) {
     if (g->exec) (void)hipGraphExecDestroy(g->exec);
     if (g->graph) (void)hipGraphDestroy(g->graph);
     delete g;
+    return OPE_HIP_OK;
+}
+
+// Stores the launch variant of the context's current configuration.
+void store_tuning(ope_hip_ctx* c, int mode, int iw, int as, int sched, int per_cu) {
+    for (auto& t : c->tuning)
+        if (t.taps == c->taps && t.mode == mode && t.iw == iw) {
+            t.as = as;
+            t.sched = sched;
+            t.per_cu = per_cu;
+            return;
+        }
+    c->tuning.push_back({c->taps, mode, iw, as, sched, per_cu});
+}
+
+int ope_hip_set_launch_variant(ope_hip_ctx* c, int ascans_per_lane, int small_radix_pass, int workgroups_per_cu) {
+    if (!c) return OPE_HIP_EINVAL;
+    std::lock_guard<std::mutex> lk(c->submit_mu);
+    if (ascans_per_lane != 1 && ascans_per_lane != 2)
+        return fail(c, OPE_HIP_EINVAL, "A-scans per lane must be 1 or 2");
+    if (workgroups_per_cu < 0) return fail(c, OPE_HIP_EINVAL, "workgroups per CU must be >= 0");
+    const int np = fft_passes(c->n);
+    if (small_radix_pass < 0 || small_radix_pass >= np)
+        return fail(c, OPE_HIP_EINVAL, "small-radix pass must be in 0..%d", np - 1);
+    const int mode = c->p.dc_removal_enabled ? M_DC : 0;
+    const int iw = (int)input_bytes(c->p.input_type);
+    const int sched = small_radix_pass == np - 1 ? -1 : small_radix_pass;
+    const ChainLaunch l = pick_kernel(c->n, c->taps, mode, iw, ascans_per_lane, sched);
+    // placements other than the last exist for one A-scan per lane only
+    if (!l.fn || (ascans_per_lane == 2 && sched != -1))
+        return fail(c, OPE_HIP_EUNSUP, "launch variant not built");
+    store_tuning(c, mode, iw, ascans_per_lane, sched, workgroups_per_cu);
+    return OPE_HIP_OK;
+}
+
+int ope_hip_autotune(ope_hip_ctx* c, const void* d_in, float* d_out, int64_t n_ascans, void* stream, int* choice,
+                     float* best_us) {
+    if (!c) return OPE_HIP_EINVAL;
+    std::lock_guard<std::mutex> lk(c->submit_mu);
+    HIP_TRY(c, hipSetDevice(c->device));
+    const ope_hip_params& p = c->p;
+    if (!d_in || !d_out) {
+        // the context's own first frame slot
+        if (c->slots.empty()) return fail(c, OPE_HIP_ESTATE, "no frame slot");
+        d_in = c->slots[0].d_in;
+        d_out = c->slots[0].d_out;
+        n_ascans = (long long)p.ascans_per_bscan * p.bscans_per_buffer;
+    }
+    if (n_ascans < 1) return fail(c, OPE_HIP_EINVAL, "n_ascans must be >= 1");
+    if (reinterpret_cast<uintptr_t>(d_in) % 16 || reinterpret_cast<uintptr_t>(d_out) % 4)
+        return fail(c, OPE_HIP_EINVAL, "d_in must be 16-byte and d_out 4-byte aligned");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    // the launch a steady-state frame makes (run_frame)
+    ChainArgs a = base_args(c);
+    a.in = d_in;
+    a.out = d_out;
+    a.n_ascans = n_ascans;
+    if (p.fpn_removal && c->have_fpn) a.flags |= F_FPN;
+    if (p.bscan_flip) {
+        a.flags |= F_FLIP;
+        a.flip_bscans = n_ascans / p.ascans_per_bscan;
+    }
+    if (p.bg_removal && c->have_bg && !p.sinusoidal_correction) a.flags |= F_BG;
+    const int mode = p.dc_removal_enabled ? M_DC : 0;
+    const int iw = (int)input_bytes(p.input_type);
+    struct Cand {
+        int as, sched, per_cu;
+    };
+    std::vector<Cand> cands;
+    const int np = fft_passes(c->n);
+    for (int as = 1; as <= 2; ++as) {
+        const int scheds = (as == 1 && (c->n >> (3 * (np - 1))) != 8) ? np : 1;
+        for (int sc = 0; sc < scheds; ++sc) {
+            const int sched = scheds == 1 ? -1 : sc;
+            const ChainLaunch l = pick_kernel(c->n, c->taps, mode, iw, as, sched);
+            if (!l.fn) continue;
+            int occ = 0;
+            HIP_TRY(c, cached_occupancy(c, reinterpret_cast<const void*>(l.fn), l.wg, &occ));
+            for (int k = 1; k <= occ; ++k) cands.push_back({as, sched, k});
+        }
+    }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIP_TRY(c, hipEventCreate(&e0));
+    HIP_TRY(c, hipEventCreate(&e1));
+    float best = 0.f;
+    Cand pick{1, -1, 0};
+    int rc = OPE_HIP_OK;
+    constexpr int kReps = 3;
+    for (const Cand& cd : cands) {
+        rc = launch_variant(c, a, mode, cd.as, cd.sched, cd.per_cu, n_ascans, s);  // warm-up
+        if (rc) break;
+        hipError_t e = hipEventRecord(e0, s);
+        for (int r = 0; r < kReps && e == hipSuccess && !rc; ++r)
+            rc = launch_variant(c, a, mode, cd.as, cd.sched, cd.per_cu, n_ascans, s);
+        if (!rc && e == hipSuccess) e = hipEventRecord(e1, s);
+        if (!rc && e == hipSuccess) e = hipEventSynchronize(e1);
+        float ms = 0.f;
+        if (!rc && e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+        if (!rc && e != hipSuccess) rc = fail(c, OPE_HIP_ERUNTIME, "autotune timing: %s", hipGetErrorString(e));
+        if (rc) break;
+        const float us = ms * 1000.f / kReps;
+        if (best == 0.f || us < best) {
+            best = us;
+            pick = cd;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    store_tuning(c, mode, iw, pick.as, pick.sched, pick.per_cu);
+    if (choice) {
+        choice[0] = pick.as;
+        choice[1] = pick.sched < 0 ? np - 1 : pick.sched;
+        choice[2] = pick.per_cu;
+    }
+    if (best_us) *best_us = best;
     return OPE_HIP_OK;
 }
 
