@@ -245,6 +245,35 @@ OPE_HIP_API int ope_hip_graph_launch(ope_hip_graph* graph, void* stream);
 OPE_HIP_API const char* ope_hip_graph_last_error(const ope_hip_graph* graph);
 OPE_HIP_API int ope_hip_graph_destroy(ope_hip_graph* graph);
 
+/* ---- Individual operations (ProcessingBackend, backend_interface.h:40-165)
+ * with the CPU backend's semantics (cpu_backend.cpp:514-733); the CUDA
+ * backend returns nothing for them (cuda_backend.cu:1133-1296).  Host
+ * pointers in and out, synchronous, run on the context's GPU.
+ *  convert_input: samples as the chain converts them (bitshift honoured;
+ *    FLOAT32 read as IEEE) -- convertInput;
+ *  klinearization: one line of line_width samples resampled at curve[j]
+ *    (LINEAR/CUBIC/LANCZOS), first `samples` outputs -- kLinearization;
+ *  windowing: input * window -- windowing;
+ *  dispersion: real part of input * (cos, sin), cos_sin interleaved --
+ *    dispersionCompensation;
+ *  ifft_magnitude: |unnormalised inverse FFT| of `lines` real lines of
+ *    line_width (a power of two 256..4096) samples -- ifft;
+ *  fpn_removal: |x - mean| of line_width * lines interleaved complex values,
+ *    the mean subtracted from the first min(total/2, line_width) values as
+ *    the reference does on the flattened lines -- fixedPatternNoiseRemoval. */
+OPE_HIP_API int ope_hip_op_convert_input(ope_hip_ctx* ctx, const void* input, int input_type, int64_t samples,
+                                         int bitshift, float* out);
+OPE_HIP_API int ope_hip_op_klinearization(ope_hip_ctx* ctx, const float* input, const float* curve, int method,
+                                          int line_width, int samples, float* out);
+OPE_HIP_API int ope_hip_op_windowing(ope_hip_ctx* ctx, const float* input, const float* window, int line_width,
+                                     int samples, float* out);
+OPE_HIP_API int ope_hip_op_dispersion(ope_hip_ctx* ctx, const float* input, const float* cos_sin, int line_width,
+                                      int samples, float* out);
+OPE_HIP_API int ope_hip_op_ifft_magnitude(ope_hip_ctx* ctx, const float* input, int line_width, int lines,
+                                          float* out);
+OPE_HIP_API int ope_hip_op_fpn_removal(ope_hip_ctx* ctx, const float* input, const float* mean, int line_width,
+                                       int lines, float* out);
+
 /* ---- Ragged batches.  No counterpart in the reference, whose backends
  * process one fixed-size frame per call (ProcessingBackend::process,
  * backend_interface.h:27).  A ragged batch runs many requests of different

@@ -48,6 +48,12 @@ SIGNATURES = {
                                     c_void_p]),
     "ope_hip_synchronize": (c_int, [c_void_p]),
     "ope_hip_op_min_variance_mean": (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_void_p]),
+    "ope_hip_op_convert_input": (c_int, [c_void_p, c_void_p, c_int, c_int64, c_int, c_void_p]),
+    "ope_hip_op_klinearization": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_void_p]),
+    "ope_hip_op_windowing": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p]),
+    "ope_hip_op_dispersion": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p]),
+    "ope_hip_op_ifft_magnitude": (c_int, [c_void_p, c_void_p, c_int, c_int, c_void_p]),
+    "ope_hip_op_fpn_removal": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p]),
     "ope_hip_host_alloc": (c_int, [c_size_t, POINTER(c_void_p)]),
     "ope_hip_host_free": (c_int, [c_void_p]),
     "ope_hip_autofocus_loss": (c_int, [c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_void_p]),

@@ -277,10 +277,11 @@ const std::vector<float>& HipBackend::getFixedPatternNoiseProfile() const {
 
 // ---- individual operations
 //
-// getMinimumVarianceMean runs on the GPU.  The other operations return empty
-// vectors, as the CUDA backend's do (cuda_backend.cu:1133-1296): they are
-// per-stage debugging hooks of the reference, not part of the processing
-// path.
+// Individual operations with the CPU backend's semantics run on the GPU
+// (convertInput, kLinearization, windowing, dispersionCompensation, ifft,
+// getMinimumVarianceMean, fixedPatternNoiseRemoval: cpu_backend.cpp:514-733).
+// The ones that are stubs in the CPU backend return empty vectors, as every
+// operation of the CUDA backend does (cuda_backend.cu:1133-1296).
 
 std::vector<float> HipBackend::getMinimumVarianceMean(const float* input, int width, int height, int segments) {
     if (!input) return {};
@@ -290,11 +291,63 @@ std::vector<float> HipBackend::getMinimumVarianceMean(const float* input, int wi
     return out;
 }
 
-std::vector<float> HipBackend::convertInput(const void*, IOBuffer::DataType, int, int, bool) { return {}; }
+std::vector<float> HipBackend::convertInput(const void* input, IOBuffer::DataType inputType, int, int samples,
+                                            bool applyBitshift) {
+    if (!input || samples <= 0) return {};
+    requireContext("convertInput");
+    std::vector<float> out(static_cast<size_t>(samples));
+    check(ope_hip_op_convert_input(ctx_, input, static_cast<int>(inputType), samples, applyBitshift ? 1 : 0,
+                                   out.data()),
+          "convertInput");
+    return out;
+}
+
+std::vector<float> HipBackend::kLinearization(const float* input, const float* resampleCurve,
+                                              InterpolationMethod method, int lineWidth, int samples) {
+    if (!input || !resampleCurve || samples <= 0) return {};
+    requireContext("kLinearization");
+    std::vector<float> out(static_cast<size_t>(samples));
+    check(ope_hip_op_klinearization(ctx_, input, resampleCurve, static_cast<int>(method), lineWidth, samples,
+                                    out.data()),
+          "kLinearization");
+    return out;
+}
+
+std::vector<float> HipBackend::windowing(const float* input, const float* windowCurve, int lineWidth, int samples) {
+    if (!input || !windowCurve || samples <= 0) return {};
+    requireContext("windowing");
+    std::vector<float> out(static_cast<size_t>(samples));
+    check(ope_hip_op_windowing(ctx_, input, windowCurve, lineWidth, samples, out.data()), "windowing");
+    return out;
+}
+
+std::vector<float> HipBackend::dispersionCompensation(const float* input, const float* phaseComplex, int lineWidth,
+                                                      int samples) {
+    if (!input || !phaseComplex || samples <= 0) return {};
+    requireContext("dispersionCompensation");
+    std::vector<float> out(static_cast<size_t>(samples));
+    check(ope_hip_op_dispersion(ctx_, input, phaseComplex, lineWidth, samples, out.data()), "dispersionCompensation");
+    return out;
+}
+
+std::vector<float> HipBackend::ifft(const float* input, int lineWidth, int) {
+    if (!input || lineWidth <= 0) return {};
+    requireContext("ifft");
+    std::vector<float> out(static_cast<size_t>(lineWidth));
+    check(ope_hip_op_ifft_magnitude(ctx_, input, lineWidth, 1, out.data()), "ifft");
+    return out;
+}
+
+std::vector<float> HipBackend::fixedPatternNoiseRemoval(const float* input, const float* meanALine, int lineWidth,
+                                                        int numLines) {
+    if (!input) return {};
+    requireContext("fixedPatternNoiseRemoval");
+    std::vector<float> out(static_cast<size_t>(std::max(lineWidth, 0)) * std::max(numLines, 0));
+    check(ope_hip_op_fpn_removal(ctx_, input, meanALine, lineWidth, numLines, out.data()), "fixedPatternNoiseRemoval");
+    return out;
+}
+
 std::vector<float> HipBackend::rollingAverageBackgroundRemoval(const float*, int, int, int) { return {}; }
-std::vector<float> HipBackend::kLinearization(const float*, const float*, InterpolationMethod, int, int) { return {}; }
-std::vector<float> HipBackend::windowing(const float*, const float*, int, int) { return {}; }
-std::vector<float> HipBackend::dispersionCompensation(const float*, const float*, int, int) { return {}; }
 std::vector<float> HipBackend::kLinearizationAndWindowing(const float*, const float*, const float*,
                                                           InterpolationMethod, int, int) {
     return {};
@@ -308,8 +361,6 @@ std::vector<float> HipBackend::dispersionCompensationAndWindowing(const float*, 
     return {};
 }
 std::vector<float> HipBackend::fft(const float*, int, int) { return {}; }
-std::vector<float> HipBackend::ifft(const float*, int, int) { return {}; }
-std::vector<float> HipBackend::fixedPatternNoiseRemoval(const float*, const float*, int, int) { return {}; }
 std::vector<float> HipBackend::postProcessTruncate(const float*, bool, float, float, float, float, int, int) {
     return {};
 }

@@ -66,7 +66,6 @@ __global__ void __launch_bounds__(Geo<N>::WG) autofocus_kernel(AutofocusArgs A) 
     float2* const pb[1] = {bufB + slot * G::PADN};
     float* const xs = L::X_ALIAS_B ? reinterpret_cast<float*>(pb[0]) : (xsep + slot * N);
     const bool ieee = A.in_type == IN_F32;
-    const int rbase = pad_idx(t);
 
     cf tw[NP][8];
     load_twiddles<N>(A.twiddle, t, tw);
@@ -135,7 +134,7 @@ __global__ void __launch_bounds__(Geo<N>::WG) autofocus_kernel(AutofocusArgs A) 
             dj[e] = d;
         }
 
-        stockham_inverse<N, 1>(v, pa, pb, tw, rbase);
+        stockham_inverse<N, 1>(v, pa, pb, tw);
         // v[0][e] = X[t + e*TPA]; bins e < 4 are k < N/2
 
         if constexpr (!BWD) {
@@ -160,7 +159,7 @@ __global__ void __launch_bounds__(Geo<N>::WG) autofocus_kernel(AutofocusArgs A) 
                 }
             }
             __syncthreads();  // the forward transform's last reads of pa/pb are done
-            stockham_inverse<N, 1>(v, pa, pb, tw, rbase);
+            stockham_inverse<N, 1>(v, pa, pb, tw);
             if (active) {
 #pragma unroll
                 for (int e = 0; e < 8; ++e) {

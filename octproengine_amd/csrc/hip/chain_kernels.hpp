@@ -91,6 +91,17 @@ __host__ __device__ constexpr int ilog2(int n) { return n <= 1 ? 0 : 1 + ilog2(n
 // family it gave the fewest SQ_LDS_BANK_CONFLICT cycles on gfx950.
 __host__ __device__ constexpr int pad_slots(int i) { return i + 2 * (i >> 5) + (i >> 7); }
 
+// Padding of the buffer written by Stockham pass 1 (and read by pass 2).
+// Pass 1 writes runs of 8 consecutive slots 64 apart; with pad_slots two of
+// every eight such runs meet in the same LDS banks (64 banks of 4 bytes on
+// gfx950), with this padding none do.  Same family, so the same base-plus-
+// immediate split applies.  Chosen by counting bank conflicts of the
+// kernel's access pattern for every length and pass.
+__host__ __device__ constexpr int pad_pass1(int i) { return i + 2 * (i >> 5) + 4 * (i >> 6); }
+
+// padding of the buffer written by pass p
+__host__ __device__ constexpr int pad_of_pass(int p, int i) { return p == 1 ? pad_pass1(i) : pad_slots(i); }
+
 template <int N>
 struct Geo {
     static constexpr int LOG2N = ilog2(N);
@@ -99,15 +110,17 @@ struct Geo {
     static constexpr int APW = WG / TPA;                    // A-scans per chain_kernel workgroup
     static constexpr int NPASS = (LOG2N + 2) / 3;           // Stockham passes
     static constexpr int LAST_R = N >> (3 * (NPASS - 1));   // radix of the one non-radix-8 pass
-    // padded float2 slots; at least N+1 so buffer A also holds the N+1
-    // prefix sums (int or double) of the DC-removal prologue
-    static constexpr int PADN = pad_slots(N - 1) + 1 > N + 1 ? pad_slots(N - 1) + 1 : N + 1;
+    // padded float2 slots (the larger of the two paddings); at least N+1 so
+    // buffer A also holds the N+1 prefix sums (int or double) of the
+    // DC-removal prologue
+    static constexpr int PADN_0 = pad_slots(N - 1) + 1 > pad_pass1(N - 1) + 1 ? pad_slots(N - 1) + 1
+                                                                               : pad_pass1(N - 1) + 1;
+    static constexpr int PADN = PADN_0 > N + 1 ? PADN_0 : N + 1;
     static constexpr int HALF = N / 2;
     static_assert((1 << LOG2N) == N, "N must be a power of two");
     static_assert(N >= 256 && N <= 4096, "N outside 256..4096");
 };
 
-__device__ __forceinline__ int pad_idx(int i) { return pad_slots(i); }
 
 // ------------------------------------------------------------------ arguments
 
@@ -342,12 +355,14 @@ __device__ __forceinline__ void load_twiddles(const float2* table, int t, cf (&t
 // flight in other wavefronts.
 template <int N, int AS, int SCHED = -1>
 __device__ __forceinline__ void stockham_inverse(cf (&v)[AS][8], float2* const (&pa)[AS], float2* const (&pb)[AS],
-                                                 const cf (&tw)[Geo<N>::NPASS][8], int rbase) {
+                                                 const cf (&tw)[Geo<N>::NPASS][8]) {
     using G = Geo<N>;
     using P = FftPlan<N, SCHED>;
     constexpr int NP = G::NPASS;
     constexpr int TPA = G::TPA;
     const int t = threadIdx.x % TPA;
+    // lane bases of the unit-stride reads, per padding
+    const int rbase0 = pad_slots(t), rbase1 = pad_pass1(t);
 #pragma unroll
     for (int p = 0; p < NP; ++p) {
         const int R = P::radix(p);
@@ -357,12 +372,12 @@ __device__ __forceinline__ void stockham_inverse(cf (&v)[AS][8], float2* const (
 #pragma unroll
             for (int u = 0; u < AS; ++u) {
                 // pass p>0 reads what pass p-1 wrote
-                const float2* rp = ((p % 2 == 1) ? pa[u] : pb[u]) + rbase;
+                const float2* rp = ((p % 2 == 1) ? pa[u] : pb[u]) + (p == 2 ? rbase1 : rbase0);
 #pragma unroll
                 for (int e = 0; e < 8; ++e) {
                     // t < TPA and e*TPA share no bits, so the padded
                     // index splits into a lane base plus a constant
-                    float2 w = rp[pad_slots(e * TPA)];
+                    float2 w = rp[pad_of_pass(p - 1, e * TPA)];
                     v[u][e] = {w.x, w.y};
                 }
             }
@@ -395,13 +410,13 @@ __device__ __forceinline__ void stockham_inverse(cf (&v)[AS][8], float2* const (
 #pragma unroll
                 for (int q = 0; q < RQ; ++q) {
                     const int b = t + q * TPA;
-                    float2* wp = dst + pad_idx((b / ns) * ns * R + (b & (ns - 1)));
+                    float2* wp = dst + pad_of_pass(p, (b / ns) * ns * R + (b & (ns - 1)));
 #pragma unroll
                     for (int m = 0; m < R; ++m) {
                         const int e = q + RQ * m;
                         // base has no bits where m*ns has any: the padded
                         // index splits into base and a constant
-                        wp[pad_slots(m * ns)] = make_float2(v[u][e].x, v[u][e].y);
+                        wp[pad_of_pass(p, m * ns)] = make_float2(v[u][e].x, v[u][e].y);
                     }
                 }
             }
@@ -558,7 +573,6 @@ __device__ __forceinline__ void chain_body(const ChainArgs A, float2* bufA, floa
     }
 
     // LDS base: every pass reads positions t + e*TPA (base + constant)
-    const int rbase = pad_idx(t);
 
     double sx[AS][4], sy[AS][4], sxx[AS][4];
     int cur_seg[AS];
@@ -624,7 +638,7 @@ __device__ __forceinline__ void chain_body(const ChainArgs A, float2* bufA, floa
         }
 
         // ---- inverse FFT: NP Stockham passes
-        stockham_inverse<N, AS, SCHED>(v, pa, pb, tw, rbase);
+        stockham_inverse<N, AS, SCHED>(v, pa, pb, tw);
         // after the last pass v[u][e] holds X[t + e*TPA]; e < 4 are the kept bins
 
 #pragma unroll

@@ -6,6 +6,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "chain_kernels.hpp"
+
 namespace ope_hip {
 
 // ------------------------------------------------------------------ FPN select
@@ -111,6 +113,90 @@ __global__ void sinusoidal_kernel(const float* __restrict__ in, float* __restric
     }
     if (use_bg) v = v - __fadd_rn(__fmul_rn(bg_w, bg[j]), bg_off);
     out[i] = v;
+}
+
+// ------------------------------------------------------------------ individual operations
+
+// Samples to float, as the chain converts them (one sample per thread).
+__global__ void op_convert_kernel(const void* __restrict__ in, float* __restrict__ out, long long samples, int iw,
+                                  int ieee, int shift) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= samples) return;
+    float v;
+    if (iw == 1) {
+        const unsigned x = static_cast<const uint8_t*>(in)[i];
+        v = (float)(shift ? x >> 4 : x);
+    } else if (iw == 2) {
+        const unsigned x = static_cast<const uint16_t*>(in)[i];
+        v = (float)(shift ? x >> 4 : x);
+    } else {
+        const uint32_t x = static_cast<const uint32_t*>(in)[i];
+        v = ieee ? __uint_as_float(x) : (shift ? (float)x * 2.3283064365386963e-10f : (float)x);
+    }
+    out[i] = v;
+}
+
+// y[j] = sum_i w[j][i] x[base[j] + i], j < count (banded tap table).
+__global__ void op_taps_kernel(const float* __restrict__ x, const int* __restrict__ base, const float* __restrict__ w,
+                               int taps, int count, float* __restrict__ out) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= count) return;
+    float acc = 0.f;
+    for (int i = 0; i < taps; ++i) acc = fmaf(w[j * taps + i], x[base[j] + i], acc);
+    out[j] = acc;
+}
+
+// out[i] = in[i] * s[i * stride], i < count (window: stride 1; real part of
+// the dispersion product with an interleaved cos/sin curve: stride 2).
+__global__ void op_scale_kernel(const float* __restrict__ in, const float* __restrict__ s, int stride, int count,
+                                float* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) out[i] = in[i] * s[(long long)i * stride];
+}
+
+// |unnormalised inverse FFT| of real lines of N samples: all N bins.
+template <int N>
+__global__ void __launch_bounds__(Geo<N>::WG) op_ifft_mag_kernel(const float* __restrict__ in, long long lines,
+                                                                 const float2* __restrict__ twiddle,
+                                                                 float* __restrict__ out) {
+    using G = Geo<N>;
+    using L = ChainLds<N, 0, 4, 1, G::WG>;
+    constexpr int TPA = G::TPA;
+    __shared__ float2 bufA[L::BUF];
+    __shared__ float2 bufB[L::BUF];
+    const int slot = threadIdx.x / TPA;
+    const int t = threadIdx.x - slot * TPA;
+    float2* const pa[1] = {bufA + slot * G::PADN};
+    float2* const pb[1] = {bufB + slot * G::PADN};
+    cf tw[G::NPASS][8];
+    load_twiddles<N>(twiddle, t, tw);
+    const long long line = (long long)blockIdx.x * L::SLOTS + slot;
+    const bool active = line < lines;
+    cf v[1][8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) v[0][e] = {active ? in[line * N + t + e * TPA] : 0.f, 0.f};
+    stockham_inverse<N, 1>(v, pa, pb, tw);
+    if (active) {
+#pragma unroll
+        for (int e = 0; e < 8; ++e)
+            out[line * N + t + e * TPA] = __fsqrt_rn(fmaf(v[0][e].x, v[0][e].x, v[0][e].y * v[0][e].y));
+    }
+}
+
+// |x - mean| over interleaved complex x of `total` values, where the mean is
+// subtracted from the first `use` values only (meanALineSubtraction applied
+// to the flattened lines, cpu_kernels.tpp:461-479 via
+// cpu_backend.cpp:708-733).
+__global__ void op_fpn_mag_kernel(const float2* __restrict__ in, const float2* __restrict__ mean, long long total,
+                                  int use, float* __restrict__ out) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    float2 z = in[i];
+    if (i < use) {
+        z.x -= mean[i].x;
+        z.y -= mean[i].y;
+    }
+    out[i] = __fsqrt_rn(z.x * z.x + z.y * z.y);
 }
 
 }  // namespace ope_hip

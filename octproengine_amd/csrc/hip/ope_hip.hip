@@ -212,17 +212,17 @@ double lanczos8(double d) {
 // folded in.  Index clamping follows cpu_kernels.tpp:84-196 (upper clamp to
 // N-1 everywhere, lower clamp to 0 for Lanczos); indices below 0, which the
 // reference reads out of bounds, are clamped to 0 here.
-int build_taps(ope_hip_ctx* c, std::vector<int>& base, std::vector<float>& w) {
-    const int n = c->n, T = c->taps;
-    const bool win = c->p.windowing_enabled != 0;
-    base.assign(n, 0);
-    w.assign((size_t)n * T, 0.f);
-    const float* curve = c->curve_resample.data();
-    const float* window = c->curve_window.data();
+// Tap table of `count` outputs over an input line of n samples with taps T
+// (1: identity) at positions curve[j], window (may be null: ones) folded in.
+int build_taps_for(ope_hip_ctx* c, int n, int count, int T, const float* curve, const float* window,
+                   std::vector<int>& base, std::vector<float>& w) {
+    const bool win = window != nullptr;
+    base.assign(count, 0);
+    w.assign((size_t)count * T, 0.f);
     std::vector<double> acc(T);
     std::vector<int> idx(16);
     std::vector<double> cw(16);
-    for (int j = 0; j < n; ++j) {
+    for (int j = 0; j < count; ++j) {
         int cnt = 0;
         if (T == 1) {
             idx[0] = j;
@@ -280,6 +280,11 @@ int build_taps(ope_hip_ctx* c, std::vector<int>& base, std::vector<float>& w) {
         for (int i = 0; i < T; ++i) w[(size_t)j * T + i] = (float)(acc[i] * wj);
     }
     return OPE_HIP_OK;
+}
+
+int build_taps(ope_hip_ctx* c, std::vector<int>& base, std::vector<float>& w) {
+    return build_taps_for(c, c->n, c->n, c->taps, c->curve_resample.data(),
+                          c->p.windowing_enabled ? c->curve_window.data() : nullptr, base, w);
 }
 
 int upload_taps(ope_hip_ctx* c) {
@@ -799,6 +804,53 @@ int build_ragged(ope_hip_ragged* r, ope_hip_ctx* const* ctxs, int n_ctx, const o
     return OPE_HIP_OK;
 }
 
+// ---- individual operations: device scratch per call, synchronous
+
+namespace {
+
+struct DevBuf {
+    void* p = nullptr;
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+};
+
+#define OP_TRY(expr)                                                                          \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            return fail(c, OPE_HIP_ERUNTIME, "%s: %s", #expr, hipGetErrorString(e_));         \
+    } while (0)
+
+int op_scale(ope_hip_ctx* c, const float* input, const float* s, int stride, int line_width, int samples, float* out) {
+    if (!c || !input || !s || !out || line_width < 1 || samples < 0 || samples > line_width) return OPE_HIP_EINVAL;
+    if (samples == 0) return OPE_HIP_OK;
+    std::lock_guard<std::mutex> lk(c->submit_mu);
+    OP_TRY(hipSetDevice(c->device));
+    DevBuf dx, ds, dy;
+    OP_TRY(hipMalloc(&dx.p, samples * sizeof(float)));
+    OP_TRY(hipMalloc(&ds.p, (size_t)samples * stride * sizeof(float)));
+    OP_TRY(hipMalloc(&dy.p, samples * sizeof(float)));
+    OP_TRY(hipMemcpy(dx.p, input, samples * sizeof(float), hipMemcpyHostToDevice));
+    OP_TRY(hipMemcpy(ds.p, s, (size_t)samples * stride * sizeof(float), hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(op_scale_kernel, dim3((samples + 255) / 256), dim3(256), 0, c->s_comp,
+                       static_cast<const float*>(dx.p), static_cast<const float*>(ds.p), stride, samples,
+                       static_cast<float*>(dy.p));
+    OP_TRY(hipGetLastError());
+    OP_TRY(hipStreamSynchronize(c->s_comp));
+    OP_TRY(hipMemcpy(out, dy.p, samples * sizeof(float), hipMemcpyDeviceToHost));
+    return OPE_HIP_OK;
+}
+
+template <int N>
+void launch_ifft_mag(const float* in, long long lines, const float2* tw, float* out, hipStream_t s) {
+    using L = ChainLds<N, 0, 4, 1, Geo<N>::WG>;
+    hipLaunchKernelGGL(op_ifft_mag_kernel<N>, dim3((unsigned)((lines + L::SLOTS - 1) / L::SLOTS)), dim3(Geo<N>::WG),
+                       0, s, in, lines, tw, out);
+}
+
+}  // namespace
+
 }  // namespace
 
 // ====================================================================== C ABI
@@ -1152,6 +1204,128 @@ int ope_hip_op_min_variance_mean(ope_hip_ctx* c, const float* input, int width, 
     cleanup();
     return rc;
 }
+
+
+int ope_hip_op_convert_input(ope_hip_ctx* c, const void* input, int input_type, int64_t samples, int bitshift,
+                             float* out) {
+    if (!c || (samples > 0 && (!input || !out)) || samples < 0) return OPE_HIP_EINVAL;
+    const int cls = input_class(input_type);
+    if (!cls) return fail(c, OPE_HIP_EUNSUP, "input type %d not supported", input_type);
+    if (samples == 0) return OPE_HIP_OK;
+    std::lock_guard<std::mutex> lk(c->submit_mu);
+    OP_TRY(hipSetDevice(c->device));
+    const size_t iw = input_bytes(input_type);
+    DevBuf din, dout;
+    OP_TRY(hipMalloc(&din.p, samples * iw));
+    OP_TRY(hipMalloc(&dout.p, samples * sizeof(float)));
+    OP_TRY(hipMemcpy(din.p, input, samples * iw, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(op_convert_kernel, dim3((unsigned)((samples + 255) / 256)), dim3(256), 0, c->s_comp, din.p,
+                       static_cast<float*>(dout.p), (long long)samples, (int)iw, cls == IN_F32 ? 1 : 0, bitshift ? 1 : 0);
+    OP_TRY(hipGetLastError());
+    OP_TRY(hipStreamSynchronize(c->s_comp));
+    OP_TRY(hipMemcpy(out, dout.p, samples * sizeof(float), hipMemcpyDeviceToHost));
+    return OPE_HIP_OK;
+}
+
+int ope_hip_op_klinearization(ope_hip_ctx* c, const float* input, const float* curve, int method, int line_width,
+                              int samples, float* out) {
+    if (!c || !input || !curve || !out || line_width < 1 || samples < 0 || samples > line_width) return OPE_HIP_EINVAL;
+    const int T = method == OPE_HIP_INTERP_LINEAR ? 2 : method == OPE_HIP_INTERP_CUBIC ? 4 : method == OPE_HIP_INTERP_LANCZOS ? 16 : 0;
+    if (!T) return fail(c, OPE_HIP_EINVAL, "unknown interpolation method %d", method);
+    if (line_width < T) return fail(c, OPE_HIP_EINVAL, "line shorter than the interpolation kernel");
+    if (samples == 0) return OPE_HIP_OK;
+    std::vector<int> base;
+    std::vector<float> w;
+    std::lock_guard<std::mutex> lk(c->submit_mu);
+    int rc = build_taps_for(c, line_width, samples, T, curve, nullptr, base, w);
+    if (rc) return rc;
+    OP_TRY(hipSetDevice(c->device));
+    DevBuf dx, db, dw, dy;
+    OP_TRY(hipMalloc(&dx.p, line_width * sizeof(float)));
+    OP_TRY(hipMalloc(&db.p, samples * sizeof(int)));
+    OP_TRY(hipMalloc(&dw.p, w.size() * sizeof(float)));
+    OP_TRY(hipMalloc(&dy.p, samples * sizeof(float)));
+    OP_TRY(hipMemcpy(dx.p, input, line_width * sizeof(float), hipMemcpyHostToDevice));
+    OP_TRY(hipMemcpy(db.p, base.data(), samples * sizeof(int), hipMemcpyHostToDevice));
+    OP_TRY(hipMemcpy(dw.p, w.data(), w.size() * sizeof(float), hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(op_taps_kernel, dim3((samples + 255) / 256), dim3(256), 0, c->s_comp,
+                       static_cast<const float*>(dx.p), static_cast<const int*>(db.p), static_cast<const float*>(dw.p),
+                       T, samples, static_cast<float*>(dy.p));
+    OP_TRY(hipGetLastError());
+    OP_TRY(hipStreamSynchronize(c->s_comp));
+    OP_TRY(hipMemcpy(out, dy.p, samples * sizeof(float), hipMemcpyDeviceToHost));
+    return OPE_HIP_OK;
+}
+
+int ope_hip_op_windowing(ope_hip_ctx* c, const float* input, const float* window, int line_width, int samples,
+                         float* out) {
+    return op_scale(c, input, window, 1, line_width, samples, out);
+}
+
+int ope_hip_op_dispersion(ope_hip_ctx* c, const float* input, const float* cos_sin, int line_width, int samples,
+                          float* out) {
+    return op_scale(c, input, cos_sin, 2, line_width, samples, out);
+}
+
+int ope_hip_op_ifft_magnitude(ope_hip_ctx* c, const float* input, int line_width, int lines, float* out) {
+    if (!c || !input || !out || lines < 0) return OPE_HIP_EINVAL;
+    if (!supported_n(line_width))
+        return fail(c, OPE_HIP_EUNSUP, "line width %d not supported (powers of two 256..4096)", line_width);
+    if (lines == 0) return OPE_HIP_OK;
+    std::lock_guard<std::mutex> lk(c->submit_mu);
+    OP_TRY(hipSetDevice(c->device));
+    const size_t bytes = (size_t)line_width * lines * sizeof(float);
+    DevBuf dx, dy, dt;
+    OP_TRY(hipMalloc(&dx.p, bytes));
+    OP_TRY(hipMalloc(&dy.p, bytes));
+    std::vector<float2> tw(line_width);
+    for (int k = 0; k < line_width; ++k) {
+        const double ang = 2.0 * M_PI * (double)k / (double)line_width;
+        tw[k] = make_float2((float)std::cos(ang), (float)std::sin(ang));
+    }
+    OP_TRY(hipMalloc(&dt.p, line_width * sizeof(float2)));
+    OP_TRY(hipMemcpy(dt.p, tw.data(), line_width * sizeof(float2), hipMemcpyHostToDevice));
+    OP_TRY(hipMemcpy(dx.p, input, bytes, hipMemcpyHostToDevice));
+    const float* x = static_cast<const float*>(dx.p);
+    const float2* t = static_cast<const float2*>(dt.p);
+    float* y = static_cast<float*>(dy.p);
+    switch (line_width) {
+        case 256: launch_ifft_mag<256>(x, lines, t, y, c->s_comp); break;
+        case 512: launch_ifft_mag<512>(x, lines, t, y, c->s_comp); break;
+        case 1024: launch_ifft_mag<1024>(x, lines, t, y, c->s_comp); break;
+        case 2048: launch_ifft_mag<2048>(x, lines, t, y, c->s_comp); break;
+        default: launch_ifft_mag<4096>(x, lines, t, y, c->s_comp); break;
+    }
+    OP_TRY(hipGetLastError());
+    OP_TRY(hipStreamSynchronize(c->s_comp));
+    OP_TRY(hipMemcpy(out, dy.p, bytes, hipMemcpyDeviceToHost));
+    return OPE_HIP_OK;
+}
+
+int ope_hip_op_fpn_removal(ope_hip_ctx* c, const float* input, const float* mean, int line_width, int lines,
+                           float* out) {
+    if (!c || !input || !out || line_width < 1 || lines < 0) return OPE_HIP_EINVAL;
+    const long long total = (long long)line_width * lines;
+    if (total == 0) return OPE_HIP_OK;
+    const int use = mean ? (int)std::min<long long>(total / 2, line_width) : 0;
+    std::lock_guard<std::mutex> lk(c->submit_mu);
+    OP_TRY(hipSetDevice(c->device));
+    DevBuf dx, dm, dy;
+    OP_TRY(hipMalloc(&dx.p, total * sizeof(float2)));
+    OP_TRY(hipMalloc(&dm.p, std::max(use, 1) * sizeof(float2)));
+    OP_TRY(hipMalloc(&dy.p, total * sizeof(float)));
+    OP_TRY(hipMemcpy(dx.p, input, total * sizeof(float2), hipMemcpyHostToDevice));
+    if (use > 0) OP_TRY(hipMemcpy(dm.p, mean, use * sizeof(float2), hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(op_fpn_mag_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, c->s_comp,
+                       static_cast<const float2*>(dx.p), static_cast<const float2*>(dm.p), total, use,
+                       static_cast<float*>(dy.p));
+    OP_TRY(hipGetLastError());
+    OP_TRY(hipStreamSynchronize(c->s_comp));
+    OP_TRY(hipMemcpy(out, dy.p, total * sizeof(float), hipMemcpyDeviceToHost));
+    return OPE_HIP_OK;
+}
+
+#undef OP_TRY
 
 int ope_hip_host_alloc(size_t bytes, void** ptr) {
     if (!ptr) return OPE_HIP_EINVAL;
