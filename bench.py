@@ -299,6 +299,9 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 256 MiB read + 256 MiB write streaming kernel, the "
                          "reference point for the FETCH_SIZE / WRITE_SIZE counters")
+    ap.add_argument("--variant", default=None, metavar="AS,PASS,WG_PER_CU",
+                    help="pin the launch variant (A-scans per lane, position of the small-radix FFT pass, "
+                         "workgroups per CU) instead of autotuning it, e.g. 1,3,3 for profiler passes")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -331,7 +334,12 @@ def main():
     for i in range(2):  # FPN determined, then the next determination's statistics
         step(i)
     torch.cuda.synchronize()
-    variant, _ = ctx.autotune(ins[0].data_ptr(), outs[0].data_ptr(), frame, sp)
+    if args.variant:
+        as_, pass_, per_cu = (int(v) for v in args.variant.split(","))
+        ctx.set_launch_variant(as_, pass_, per_cu)
+        variant = {"ascans_per_lane": as_, "small_radix_pass": pass_, "workgroups_per_cu": per_cu, "pinned": True}
+    else:
+        variant, _ = ctx.autotune(ins[0].data_ptr(), outs[0].data_ptr(), frame, sp)
     t_ramp = time.perf_counter()
     ramp_frames = 0
     while time.perf_counter() - t_ramp < args.ramp_seconds:
@@ -448,7 +456,7 @@ def main():
                                          "path": "ope_hip_submit_host, 2 pinned frames in flight"}),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(),
-                     "kernel": "chain_kernel<2048, 4, 0, 2, 1>", "bytes_per_launch": algo_bytes},
+                     "kernel": f"chain_kernel<2048, 4, 0, 2, {variant['ascans_per_lane']}, {variant['small_radix_pass']}>", "bytes_per_launch": algo_bytes},
     }
     if not args.no_cpu_baseline:
         v1, done1, dt1 = cpu_baseline(cfg, args.cpu_seconds, 1)
