@@ -222,24 +222,25 @@ __device__ __forceinline__ void load_raw(const void* in, int n, long long a, int
 // the rounded value); IEEE floats are not shifted.
 template <int IW>
 __device__ __forceinline__ void convert_raw(bool ieee, bool shift, const uint32_t (&raw)[8], float (&x)[8]) {
+    // 8/16-bit fields convert with one sub-dword-select conversion each; the
+    // shift then is an exact float division by 16 and floor (v < 2^16).
     if constexpr (IW == 2) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            x[2 * i] = (float)(uint16_t)raw[i];
+            x[2 * i + 1] = (float)(uint16_t)(raw[i] >> 16);
+        }
         if (shift) {
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                x[2 * i] = (float)((raw[i] & 0xffffu) >> 4);
-                x[2 * i + 1] = (float)(raw[i] >> 20);
-            }
-        } else {
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                x[2 * i] = (float)(raw[i] & 0xffffu);
-                x[2 * i + 1] = (float)(raw[i] >> 16);
-            }
+            for (int i = 0; i < 8; ++i) x[i] = floorf(x[i] * 0.0625f);
         }
     } else if constexpr (IW == 1) {
-        const int s = shift ? 4 : 0;
 #pragma unroll
-        for (int i = 0; i < 8; ++i) x[i] = (float)(((raw[i >> 2] >> (8 * (i & 3))) & 0xffu) >> s);
+        for (int i = 0; i < 8; ++i) x[i] = (float)(uint8_t)(raw[i >> 2] >> (8 * (i & 3)));
+        if (shift) {
+#pragma unroll
+            for (int i = 0; i < 8; ++i) x[i] = floorf(x[i] * 0.0625f);
+        }
     } else {
         const float scale = (shift && !ieee) ? 2.3283064365386963e-10f : 1.0f;  // 2^-32
 #pragma unroll
@@ -519,7 +520,8 @@ __device__ __forceinline__ void chain_body(const ChainArgs A, float2* bufA, floa
     constexpr bool X_ALIAS_B = L::X_ALIAS_B;
     using DcSum = typename L::DcSum;
 
-    const int slot = threadIdx.x / TPA;
+    // one A-scan per workgroup iteration: the A-scan index is uniform
+    const int slot = (WG == TPA) ? 0 : (int)threadIdx.x / TPA;
     const int t = threadIdx.x - slot * TPA;
     float2* pa[AS];
     float2* pb[AS];
@@ -713,8 +715,8 @@ __device__ __forceinline__ void chain_body(const ChainArgs A, float2* bufA, floa
                     OPE_GLOBAL float* dst = gmem(A.out) + row * G::HALF + t;
 #pragma unroll
                     for (int o = 0; o < 4; ++o) {
-                        const float r = (flags & F_BG) ? res[o] - bgv[o] : res[o];
-                        __builtin_nontemporal_store(r, dst + o * TPA);
+                        // bgv is 0 without background subtraction: x - 0 == x exactly
+                        __builtin_nontemporal_store(res[o] - bgv[o], dst + o * TPA);
                     }
                 }
             }
