@@ -279,6 +279,60 @@ def autofocus_measurement(capi, S, torch):
             "d2_d3_found": [float(f"{th[0]:.4e}"), float(f"{th[1]:.4e}")]}
 
 
+SWEEP_METHODS = {"none": -1, "linear": 0, "cubic": 1, "lanczos": 2}
+
+
+def sweep_configs():
+    """The chain's configurations timed by --sweep: every length with each
+    k-linearization (dispersion on), then dispersion off, DC removal, and
+    the 256/512-point lengths with cubic.  Frames of 2^25 samples (16384
+    A-scans at N = 2048, as the benchmark workload)."""
+    cfgs = [(n, m, True, False) for n in (1024, 2048, 4096) for m in ("none", "linear", "cubic", "lanczos")]
+    cfgs += [(2048, "cubic", False, False), (2048, "cubic", True, True), (2048, "lanczos", True, True),
+             (256, "cubic", True, False), (512, "cubic", True, False)]
+    return cfgs
+
+
+def sweep_one(capi, S, torch, n, method, disp, dc, frames=60, ramp_s=0.5):
+    """One configuration: uint16 frames of 2^25 samples (16 B-scans), Hann
+    window, log output, autotuned launch, then `frames` timed launches."""
+    a = (1 << 25) // n
+    params = capi.make_params(
+        signal_length=n, ascans_per_bscan=a // 16, bscans_per_buffer=16, input_type=capi.UINT16,
+        resampling_enabled=int(method != "none"), interpolation=max(SWEEP_METHODS[method], 0),
+        windowing_enabled=1, dispersion_enabled=int(disp), log_scaling=1, grayscale_min=30.0,
+        grayscale_max=100.0, dc_removal_enabled=int(dc), dc_window_size=64)
+    ctx = capi.Context(params)
+    ctx.set_resampling_curve(S.resampling_curve(n))
+    ctx.set_window_curve(hann_window(n))
+    ctx.set_dispersion_curve(cos_sin(S.dispersion_phase(n, 5e-6 * (2048 / n) ** 2, -2e-9 * (2048 / n) ** 3)))
+    ins = [S.fringes_torch(n, a, seed=i, device="cuda") for i in range(4)]
+    outs = [torch.empty((a, n // 2), dtype=torch.float32, device="cuda") for _ in range(4)]
+    stream = torch.cuda.current_stream()
+    sp = stream.cuda_stream
+    variant, _ = ctx.autotune(ins[0].data_ptr(), outs[0].data_ptr(), a, sp)
+    t_end, i = time.perf_counter() + ramp_s, 0
+    while time.perf_counter() < t_end:
+        for _ in range(8):
+            ctx.process_device(ins[i % 4].data_ptr(), outs[i % 4].data_ptr(), a, sp)
+            i += 1
+        torch.cuda.synchronize()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    ev0.record(stream)
+    for i in range(frames):
+        ctx.process_device(ins[i % 4].data_ptr(), outs[i % 4].data_ptr(), a, sp)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    us = ev0.elapsed_time(ev1) * 1e3 / frames
+    assert torch.isfinite(outs[0]).all().item()
+    ctx.close()
+    algo = a * (2 * n + 4 * (n // 2))
+    return {"signal_length": n, "klinearization": method, "dispersion": disp, "dc_removal": dc,
+            "ascans_per_frame": a, "us_per_frame": round(us, 2), "ascans_per_s": round(a / us * 1e6, 1),
+            "hbm_frac": round(algo / us / 1e3 / HBM_PEAK_GBS, 4), "launch_variant": variant}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -299,6 +353,9 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 256 MiB read + 256 MiB write streaming kernel, the "
                          "reference point for the FETCH_SIZE / WRITE_SIZE counters")
+    ap.add_argument("--sweep", action="store_true",
+                    help="instead of the benchmark line, print one JSON line per chain configuration "
+                         "(lengths, k-linearizations, dispersion, DC removal), each timed on its own")
     ap.add_argument("--variant", default=None, metavar="AS,PASS,WG_PER_CU",
                     help="pin the launch variant (A-scans per lane, position of the small-radix FFT pass, "
                          "workgroups per CU) instead of autotuning it, e.g. 1,3,3 for profiler passes")
@@ -315,6 +372,11 @@ def main():
 
     from octproengine_amd import _capi as capi
     from octproengine_amd import synthetic as S
+
+    if args.sweep:
+        for c in sweep_configs():
+            print(json.dumps(sweep_one(capi, S, torch, *c)), flush=True)
+        return
 
     cfg = workload_c2()
     n = cfg["n"]

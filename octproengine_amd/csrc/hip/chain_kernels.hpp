@@ -250,27 +250,33 @@ __device__ __forceinline__ void convert_raw(bool ieee, bool shift, const uint32_
 
 // ------------------------------------------------------------------ scans
 
-// Inclusive scan of v over the TPA lanes of one A-scan.  `part` is LDS
-// scratch of WG/64 entries (one per wavefront).  Called by all lanes.
-template <typename T, int TPA>
-__device__ __forceinline__ T ascan_inclusive_scan(T v, T* part) {
-    const int lane = threadIdx.x & 63;
-    constexpr int W = TPA < 64 ? TPA : 64;
-#pragma unroll
-    for (int d = 1; d < W; d <<= 1) {
-        T o = __shfl_up(v, d, 64);
-        if ((lane % W) >= d) v += o;
-    }
-    if constexpr (TPA > 64) {
-        const int wave = threadIdx.x >> 6;
-        if (lane == 63) part[wave] = v;
-        __syncthreads();
-        const int first = (threadIdx.x / TPA) * (TPA / 64);
-        T off = T(0);
-        for (int w = first; w < wave; ++w) off += part[w];
-        __syncthreads();
-        v += off;
-    }
+// Lane v of a wavefront moved by the DPP control CTRL (rows of 16 lanes,
+// ROW_MASK selects the rows written); lanes without a source get 0.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ int dpp(int v) {
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, ROW_MASK, 0xf, false);
+}
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp(double v) {
+    // the two 32-bit halves move alike: exact
+    const long long b = __double_as_longlong(v);
+    const unsigned lo = (unsigned)dpp<CTRL, ROW_MASK>((int)b);
+    const unsigned hi = (unsigned)dpp<CTRL, ROW_MASK>((int)(b >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// Inclusive prefix sum over aligned groups of G lanes (G = 32 or 64) of a
+// wavefront: row shifts by 1, 2, 4, 8 within rows of 16 lanes, then the
+// row totals through row broadcasts (VALU only, no LDS).
+template <int G, typename T>
+__device__ __forceinline__ T group_inclusive_scan(T v) {
+    static_assert(G == 32 || G == 64, "groups of 32 or 64 lanes");
+    v += dpp<0x111, 0xf>(v);                        // row_shr:1
+    v += dpp<0x112, 0xf>(v);                        // row_shr:2
+    v += dpp<0x114, 0xf>(v);                        // row_shr:4
+    v += dpp<0x118, 0xf>(v);                        // row_shr:8
+    v += dpp<0x142, 0xa>(v);                        // row_bcast:15 into rows 1 and 3
+    if constexpr (G == 64) v += dpp<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
     return v;
 }
 
@@ -432,42 +438,94 @@ enum : int {
     M_STATS = 2,   // FPN determination: accumulate segment statistics, no output
 };
 
-// Rolling-average DC removal of the lane's 8 samples (cpu_kernels.tpp:56-82).
-// Window sums come from exact prefix sums (int for 8/16-bit input, double
-// otherwise) built in LDS scratch `P` (N+1 entries).
+// Rolling-average DC removal of lane t's samples n = 8t .. 8t+7
+// (cpu_kernels.tpp:56-82): x[n] minus the mean of x[start .. end] with
+// start = max(n - (W-1), 0), end = min(n + W, N-1) for W >= 1 and
+// start = 0, end = n for W = 0.  Window sums are differences of exact
+// inclusive prefix sums Q[n] = x[0] + ... + x[n] (N entries of S in LDS:
+// int for 8/16-bit samples, double for 32-bit ones); `part` holds one S per
+// wavefront.  The numerator x*cnt - sum is exact for 8/16-bit samples; the
+// quotient is within one ulp of the correctly rounded one (lanes whose 8
+// windows all lie inside the A-scan scale by 1/(2W), the others take a
+// reciprocal with one correction step).  Contains two workgroup barriers
+// (one for TPA <= 64); every earlier read of Q and part must be ordered
+// before the call.
 template <int N, int TPA, typename S>
-__device__ __forceinline__ void dc_remove(float (&xv)[8], int t, int window, S* P, S* part) {
-    S loc[8];
+__device__ __forceinline__ void dc_remove(float (&xv)[8], int t, int window, S* Q, S* part) {
+    // wider windows cover the whole A-scan alike; the host rejects W < 0
+    const int W = window > N ? N : (window < 0 ? 0 : window);
+    S xi[8], loc[8];
     S s = S(0);
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
-        s += (S)xv[i];
+        xi[i] = (S)xv[i];  // exact: integers below 2^16, or any float/uint32 in double
+        s += xi[i];
         loc[i] = s;
     }
-    const S incl = ascan_inclusive_scan<S, TPA>(s, part);
-    const S excl = incl - s;
-    if (t == 0) P[0] = S(0);
+    S incl = group_inclusive_scan<(TPA < 64 ? TPA : 64)>(s);
+    if constexpr (TPA > 64) {
+        constexpr int WPA = TPA / 64;  // wavefronts per A-scan
+        const int wave = threadIdx.x >> 6;
+        if ((threadIdx.x & 63) == 63) part[wave] = incl;
+        __syncthreads();
+        const int first = (threadIdx.x / TPA) * WPA;
+        S off = S(0);
 #pragma unroll
-    for (int i = 0; i < 8; ++i) P[8 * t + i + 1] = excl + loc[i];
-    __syncthreads();
-#pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        const int n = 8 * t + i;
-        int start, end;
-        if (window >= 1) {
-            start = (n >= window - 1) ? n - (window - 1) : 0;
-            end = (n + window < N - 1) ? n + window : N - 1;
-        } else {
-            start = 0;
-            end = n;
+        for (int w = 0; w < WPA - 1; ++w) {
+            const S pw = part[first + w];
+            off += (first + w < wave) ? pw : S(0);
         }
-        const S sum = P[end + 1] - P[start];
-        const int cnt = end - start + 1;
-        // x - sum/cnt with a single rounding of the result
-        if constexpr (sizeof(S) == 4)
-            xv[i] = (float)((int)xv[i] * cnt - sum) / (float)cnt;  // numerator exact
-        else
-            xv[i] = (float)((double)xv[i] - (double)sum / (double)cnt);
+        incl += off;
+    }
+    const S excl = incl - s;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) Q[8 * t + i] = excl + loc[i];
+    __syncthreads();
+    // n0 opaque to the optimiser: the window bounds below are loop-invariant
+    // in the caller's A-scan loop, and hoisted out of it they would hold ~24
+    // registers through the FFT
+    int n0 = 8 * t;
+    asm volatile("" : "+v"(n0));
+    const bool interior = W >= 1 && n0 >= W && n0 + 7 + W <= N - 2;
+    // the branch is taken per wavefront: a wavefront with any clipped window
+    // runs the general form in all its lanes, instead of both forms
+    if (__all(interior)) {
+        // start = n - W + 1, end = n + W, cnt = 2W
+        const S* hi = Q + n0 + W;
+        const S* lo = Q + n0 - W;
+        if constexpr (sizeof(S) == 4) {
+            const float rc = 1.0f / (float)(2 * W);
+#pragma unroll
+            for (int i = 0; i < 8; ++i) xv[i] = (float)(__mul24(xi[i], 2 * W) - (hi[i] - lo[i])) * rc;
+        } else {
+            const double rc = 1.0 / (double)(2 * W);
+#pragma unroll
+            for (int i = 0; i < 8; ++i) xv[i] = (float)(xi[i] - (hi[i] - lo[i]) * rc);
+        }
+    } else {
+        // end = min(n + W, N-1), start - 1 = max(n - W, -1)  (W = 0: -1 and n)
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int n = n0 + i;
+            const int end = n + W < N - 1 ? n + W : N - 1;
+            const int lo = W >= 1 ? n - W : -1;
+            const S below = Q[lo > 0 ? lo : 0];
+            const S sum = Q[end] - (lo >= 0 ? below : S(0));
+            const int cnt = end - (lo > -1 ? lo : -1);
+            if constexpr (sizeof(S) == 4) {
+                // exact numerator (24-bit factors); reciprocal and one
+                // correction step
+                const float num = (float)(__mul24(xi[i], cnt) - sum);
+                const float c = (float)cnt;
+                const float r = __builtin_amdgcn_rcpf(c);
+                const float q = num * r;
+                xv[i] = fmaf(fmaf(-q, c, num), r, q);
+            } else {
+                // exact double division: the hardware double reciprocal
+                // alone is too coarse next to sums of 32-bit samples
+                xv[i] = (float)(xi[i] - sum / (double)cnt);
+            }
+        }
     }
 }
 
@@ -493,6 +551,15 @@ struct ChainLds {
     static constexpr int X = X_ALIAS_B ? 1 : SLOTS * N;                 // floats
     static constexpr int SCAN = (MODE & M_DC) ? AS * NW : 1;            // DcSum
     static constexpr int MM = (MODE & M_STATS) ? 1 : 2 * AS * NW;       // floats
+    // DC prefix sums in an array of their own (else they alias buffer A and
+    // wait for the previous A-scan's last pass): chain_kernel's geometry
+    // with one A-scan per lane, where three workgroups still fit in the
+    // 160 KiB of LDS of a CU
+    static constexpr int BASE_BYTES = 2 * BUF * 8 + X * 4 + SCAN * (int)sizeof(DcSum) + MM * 4;
+    static constexpr int Q_BYTES = SLOTS * N * (int)sizeof(DcSum);
+    static constexpr bool Q_SEP = (MODE & M_DC) != 0 && AS == 1 && WG == G::WG &&
+                                  3 * (BASE_BYTES + Q_BYTES) <= 160 * 1024;
+    static constexpr int Q = Q_SEP ? SLOTS * N : 1;                      // DcSum
 };
 
 // The chain over A-scans a_begin + k*a_step ... (< a_end) of A, each
@@ -509,6 +576,7 @@ struct ChainLds {
 template <int N, int T, int MODE, int IW, int AS, int WG, int SCHED = -1>
 __device__ __forceinline__ void chain_body(const ChainArgs A, float2* bufA, float2* bufB, float* xsep,
                                            typename ChainLds<N, MODE, IW, AS, WG>::DcSum* scan_part,
+                                           typename ChainLds<N, MODE, IW, AS, WG>::DcSum* qsep,
                                            float* mm_part, long long a_begin, long long a_end, long long a_step) {
     using G = Geo<N>;
     using L = ChainLds<N, MODE, IW, AS, WG>;
@@ -611,13 +679,14 @@ __device__ __forceinline__ void chain_body(const ChainArgs A, float2* bufA, floa
         }
 
         if constexpr (DC) {
-            // Prefix sums go to buffer A; the previous A-scan's last pass may
-            // still be reading A, so wait for it first.
-            __syncthreads();
+            // Prefix sums aliasing buffer A: the previous A-scan's last pass
+            // may still be reading A.  With TPA > 64 the first barrier inside
+            // dc_remove already orders those reads before the writes.
+            if constexpr (!L::Q_SEP && TPA <= 64) __syncthreads();
 #pragma unroll
             for (int u = 0; u < AS; ++u) {
-                dc_remove<N, TPA, DcSum>(xv[u], t, A.dc_window, reinterpret_cast<DcSum*>(pa[u]),
-                                                scan_part + u * NW);
+                DcSum* q = L::Q_SEP ? qsep + (slot * AS + u) * N : reinterpret_cast<DcSum*>(pa[u]);
+                dc_remove<N, TPA, DcSum>(xv[u], t, A.dc_window, q, scan_part + u * NW);
                 float4* dst = reinterpret_cast<float4*>(xs[u] + 8 * t);
                 dst[0] = make_float4(xv[u][0], xv[u][1], xv[u][2], xv[u][3]);
                 dst[1] = make_float4(xv[u][4], xv[u][5], xv[u][6], xv[u][7]);
@@ -754,6 +823,7 @@ chain_kernel(ChainArgs A) {
     __shared__ float2 bufB[L::BUF];
     __shared__ float xsep[L::X];
     __shared__ typename L::DcSum scan_part[L::SCAN];
+    __shared__ typename L::DcSum qsep[L::Q];
     __shared__ float mm_part[L::MM];
     constexpr int SLOTS = L::SLOTS;
     long long a_begin, a_step, a_end;
@@ -770,7 +840,7 @@ chain_kernel(ChainArgs A) {
         a_end = A.n_ascans;
         a_step = (long long)gridDim.x * SLOTS;
     }
-    chain_body<N, T, MODE, IW, AS, WG, SCHED>(A, bufA, bufB, xsep, scan_part, mm_part, a_begin, a_end, a_step);
+    chain_body<N, T, MODE, IW, AS, WG, SCHED>(A, bufA, bufB, xsep, scan_part, qsep, mm_part, a_begin, a_end, a_step);
 }
 
 }  // namespace ope_hip

@@ -61,7 +61,9 @@ struct RaggedLds {
 template <int N, int T, int MODE, int IW>
 __device__ __attribute__((noinline)) void ragged_task(const ChainArgs& A, float2* bufA, float2* bufB, float* xsep,
         typename ChainLds<N, MODE, IW, 1, kRaggedWG>::DcSum* scan_part, float* mm_part, long long a0, long long a1) {
-    chain_body<N, T, MODE, IW, 1, kRaggedWG>(A, bufA, bufB, xsep, scan_part, mm_part, a0, a1,
+    // the ragged geometry keeps the DC prefix sums in buffer A (no separate array)
+    static_assert(!ChainLds<N, MODE, IW, 1, kRaggedWG>::Q_SEP, "ragged tasks have no separate prefix array");
+    chain_body<N, T, MODE, IW, 1, kRaggedWG>(A, bufA, bufB, xsep, scan_part, nullptr, mm_part, a0, a1,
                                              ChainLds<N, MODE, IW, 1, kRaggedWG>::SLOTS);
 }
 

@@ -93,12 +93,24 @@ def test_input_types(dtype):
     assert e <= DB_TOL
 
 
-@pytest.mark.parametrize("dtype", [np.uint16, np.float32])
-@pytest.mark.parametrize("window", [1, 64, 200])
+def _typed(raw16, dtype):
+    if dtype == np.uint8:
+        return (raw16 >> 8).astype(np.uint8)
+    if dtype == np.uint32:
+        return raw16.astype(np.uint32) * 65537
+    if dtype == np.float32:
+        return (raw16 / 65535.0).astype(np.float32)
+    return raw16
+
+
+# windows: 0 (cumulative mean), 1, odd, typical, wide, wider than half the
+# A-scan (every window clipped) and wider than the A-scan
+@pytest.mark.parametrize("dtype", [np.uint8, np.uint16, np.uint32, np.float32])
+@pytest.mark.parametrize("window", [0, 1, 7, 64, 200, 1500, 5000])
 def test_dc_removal(dtype, window):
     n, a = 2048, 19
     p = db_params(_params(n, a, "cubic", True, dc_removal=True, dc_window=window))
-    raw = S.fringes(n, a, seed=5, dtype=dtype)
+    raw = _typed(S.fringes(n, a, seed=5), dtype)
     t = standard_tables(n)
     g = GpuChain(p, t.curve, t.window, t.phasor, dtype=dtype)
     gpu = g.frame(raw)
@@ -108,6 +120,24 @@ def test_dc_removal(dtype, window):
     e, ec = db_error(gpu, ref), db_error(c32, ref)
     print(f"DC W={window} {np.dtype(dtype).name}: dB err near peak gpu={e:.2e} cpu32={ec:.2e}")
     assert e <= DB_TOL
+
+
+@pytest.mark.parametrize("n", [256, 512, 1024, 4096])
+@pytest.mark.parametrize("window", [0, 3, 64, 300])
+def test_dc_removal_lengths_linear(n, window):
+    """Every length (one and several A-scans per workgroup, one and several
+    wavefronts per A-scan) with linear output: peak-normalised error."""
+    a = 37
+    p = lin_params(_params(n, a, "cubic", True, dc_removal=True, dc_window=window))
+    raw = S.fringes(n, a, seed=n + window)
+    t = standard_tables(n)
+    g = GpuChain(p, t.curve, t.window, t.phasor)
+    gpu = g.frame(raw)
+    g.close()
+    ref = O.process_frame(raw, p, t.curve, t.window, t.phasor)
+    e = lin_error(gpu, ref)
+    print(f"DC N={n} W={window}: peak-normalised linear error {e:.2e}")
+    assert e <= LIN_TOL
 
 
 def test_grayscale_map():
