@@ -84,10 +84,10 @@ __global__ void __launch_bounds__(Geo<N>::WG) autofocus_kernel(AutofocusArgs A) 
         float xv[8];
         convert_raw<IW>(ieee, A.bitshift != 0, raw, xv);
         __syncthreads();  // the previous A-scan's LDS reads are done
-        if constexpr (DC)
-            dc_remove<N, TPA, typename L::DcSum>(xv, t, A.dc_window, reinterpret_cast<typename L::DcSum*>(pa[0]),
-                                                 scan_part);
-        {
+        if constexpr (DC) {
+            dc_remove_stage<N, TPA, typename L::DcSum>(xv, t, A.dc_window,
+                                                       reinterpret_cast<typename L::DcSum*>(pa[0]), scan_part, xs);
+        } else {
             float4* dst = reinterpret_cast<float4*>(xs + 8 * t);
             dst[0] = make_float4(xv[0], xv[1], xv[2], xv[3]);
             dst[1] = make_float4(xv[4], xv[5], xv[6], xv[7]);

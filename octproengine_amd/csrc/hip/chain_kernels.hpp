@@ -438,28 +438,37 @@ enum : int {
     M_STATS = 2,   // FPN determination: accumulate segment statistics, no output
 };
 
-// Rolling-average DC removal of lane t's samples n = 8t .. 8t+7
-// (cpu_kernels.tpp:56-82): x[n] minus the mean of x[start .. end] with
-// start = max(n - (W-1), 0), end = min(n + W, N-1) for W >= 1 and
-// start = 0, end = n for W = 0.  Window sums are differences of exact
-// inclusive prefix sums Q[n] = x[0] + ... + x[n] (N entries of S in LDS:
-// int for 8/16-bit samples, double for 32-bit ones); `part` holds one S per
-// wavefront.  The numerator x*cnt - sum is exact for 8/16-bit samples; the
-// quotient is within one ulp of the correctly rounded one (lanes whose 8
-// windows all lie inside the A-scan scale by 1/(2W), the others take a
-// reciprocal with one correction step).  Contains two workgroup barriers
-// (one for TPA <= 64); every earlier read of Q and part must be ordered
-// before the call.
+// Rolling-average DC removal of one A-scan (cpu_kernels.tpp:56-82), staged
+// into xs (N floats in LDS): sample n becomes x[n] minus the mean of
+// x[start .. end], start = max(n - (W-1), 0), end = min(n + W, N-1) for
+// W >= 1, and start = 0, end = n for W = 0.  Lane t brings samples
+// n = 8t .. 8t+7 in xv.  Window sums are differences of exact inclusive
+// prefix sums Q[n] = x[0] + ... + x[n] (N entries of S in LDS: int for
+// 8/16-bit samples, double for 32-bit ones); `part` holds one S per
+// wavefront.
+//   The prefix sums are built in the lane's own 8 consecutive samples; the
+// means are then taken with lane t on samples t + i*TPA, so that every LDS
+// access of that step is lane-consecutive (no bank conflicts) and the
+// clipped windows at both ends fall on one sample per lane (i = 0 and
+// i = 7) as long as W <= TPA - 2.  For i = 1..6 the windows lie inside the
+// A-scan (cnt = 2W): the exact integer numerator x*2W - sum is scaled by the
+// reciprocal of 2W (within one ulp); the end samples divide exactly.
+// Contains two workgroup barriers (one for TPA <= 64); every earlier read of
+// Q, part and xs must be ordered before the call.
 template <int N, int TPA, typename S>
-__device__ __forceinline__ void dc_remove(float (&xv)[8], int t, int window, S* Q, S* part) {
+__device__ __forceinline__ void dc_remove_stage(const float (&xv)[8], int t, int window, S* Q, S* part, float* xs) {
     // wider windows cover the whole A-scan alike; the host rejects W < 0
     const int W = window > N ? N : (window < 0 ? 0 : window);
-    S xi[8], loc[8];
+    {
+        float4* dst = reinterpret_cast<float4*>(xs + 8 * t);
+        dst[0] = make_float4(xv[0], xv[1], xv[2], xv[3]);
+        dst[1] = make_float4(xv[4], xv[5], xv[6], xv[7]);
+    }
+    S loc[8];
     S s = S(0);
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
-        xi[i] = (S)xv[i];  // exact: integers below 2^16, or any float/uint32 in double
-        s += xi[i];
+        s += (S)xv[i];  // exact: integers below 2^16, or any float/uint32 in double
         loc[i] = s;
     }
     S incl = group_inclusive_scan<(TPA < 64 ? TPA : 64)>(s);
@@ -481,51 +490,50 @@ __device__ __forceinline__ void dc_remove(float (&xv)[8], int t, int window, S* 
 #pragma unroll
     for (int i = 0; i < 8; ++i) Q[8 * t + i] = excl + loc[i];
     __syncthreads();
-    // n0 opaque to the optimiser: the window bounds below are loop-invariant
-    // in the caller's A-scan loop, and hoisted out of it they would hold ~24
-    // registers through the FFT
-    int n0 = 8 * t;
-    asm volatile("" : "+v"(n0));
-    const bool interior = W >= 1 && n0 >= W && n0 + 7 + W <= N - 2;
-    // the branch is taken per wavefront: a wavefront with any clipped window
-    // runs the general form in all its lanes, instead of both forms
-    if (__all(interior)) {
-        // start = n - W + 1, end = n + W, cnt = 2W
-        const S* hi = Q + n0 + W;
-        const S* lo = Q + n0 - W;
-        if constexpr (sizeof(S) == 4) {
-            const float rc = 1.0f / (float)(2 * W);
+
+    // samples t + i*TPA.  With 1 <= W <= TPA - 2, i = 1..6 are always
+    // inside the A-scan, i = 0 is inside iff t >= W, i = 7 iff
+    // t <= TPA - 2 - W.
+    const bool banded = W >= 1 && W <= TPA - 2;
+    const S* hi = Q + t + W;
+    const S* lo = Q + t - W;  // read only where t + i*TPA >= W
+    if (banded) {
 #pragma unroll
-            for (int i = 0; i < 8; ++i) xv[i] = (float)(__mul24(xi[i], 2 * W) - (hi[i] - lo[i])) * rc;
+        for (int i = 1; i < 7; ++i) {
+            const int n = t + i * TPA;
+            const S sum = hi[i * TPA] - lo[i * TPA];
+            if constexpr (sizeof(S) == 4)
+                xs[n] = (float)(__mul24((S)xs[n], 2 * W) - sum) * (1.0f / (float)(2 * W));
+            else
+                xs[n] = (float)((S)xs[n] - sum * (1.0 / (double)(2 * W)));
+        }
+    }
+    // the end samples (all samples without the band): one copy of the code,
+    // not unrolled, so that it adds few registers
+#pragma unroll 1
+    for (int k = 0; k < (banded ? 2 : 8); ++k) {
+        const int i = banded ? 7 * k : k;
+        const int n = t + i * TPA;
+        const bool inside = banded && (i > 0 || t >= W) && (i < 7 || t <= TPA - 2 - W);
+        int start, end;
+        if (inside) {
+            start = n - W + 1;
+            end = n + W;
+        } else if (W >= 1) {
+            start = (n >= W - 1) ? n - (W - 1) : 0;
+            end = (n + W < N - 1) ? n + W : N - 1;
         } else {
-            const double rc = 1.0 / (double)(2 * W);
-#pragma unroll
-            for (int i = 0; i < 8; ++i) xv[i] = (float)(xi[i] - (hi[i] - lo[i]) * rc);
+            start = 0;
+            end = n;
         }
-    } else {
-        // end = min(n + W, N-1), start - 1 = max(n - W, -1)  (W = 0: -1 and n)
-#pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            const int n = n0 + i;
-            const int end = n + W < N - 1 ? n + W : N - 1;
-            const int lo = W >= 1 ? n - W : -1;
-            const S below = Q[lo > 0 ? lo : 0];
-            const S sum = Q[end] - (lo >= 0 ? below : S(0));
-            const int cnt = end - (lo > -1 ? lo : -1);
-            if constexpr (sizeof(S) == 4) {
-                // exact numerator (24-bit factors); reciprocal and one
-                // correction step
-                const float num = (float)(__mul24(xi[i], cnt) - sum);
-                const float c = (float)cnt;
-                const float r = __builtin_amdgcn_rcpf(c);
-                const float q = num * r;
-                xv[i] = fmaf(fmaf(-q, c, num), r, q);
-            } else {
-                // exact double division: the hardware double reciprocal
-                // alone is too coarse next to sums of 32-bit samples
-                xv[i] = (float)(xi[i] - sum / (double)cnt);
-            }
-        }
+        const S sum = Q[end] - (start > 0 ? Q[start - 1] : S(0));
+        const int cnt = end - start + 1;
+        const S x = (S)xs[n];
+        // x - sum/cnt with a single rounding of the result
+        if constexpr (sizeof(S) == 4)
+            xs[n] = (float)(__mul24(x, cnt) - sum) / (float)cnt;  // numerator exact
+        else
+            xs[n] = (float)(x - sum / (double)cnt);
     }
 }
 
@@ -681,15 +689,12 @@ __device__ __forceinline__ void chain_body(const ChainArgs A, float2* bufA, floa
         if constexpr (DC) {
             // Prefix sums aliasing buffer A: the previous A-scan's last pass
             // may still be reading A.  With TPA > 64 the first barrier inside
-            // dc_remove already orders those reads before the writes.
+            // dc_remove_stage already orders those reads before the writes.
             if constexpr (!L::Q_SEP && TPA <= 64) __syncthreads();
 #pragma unroll
             for (int u = 0; u < AS; ++u) {
                 DcSum* q = L::Q_SEP ? qsep + (slot * AS + u) * N : reinterpret_cast<DcSum*>(pa[u]);
-                dc_remove<N, TPA, DcSum>(xv[u], t, A.dc_window, q, scan_part + u * NW);
-                float4* dst = reinterpret_cast<float4*>(xs[u] + 8 * t);
-                dst[0] = make_float4(xv[u][0], xv[u][1], xv[u][2], xv[u][3]);
-                dst[1] = make_float4(xv[u][4], xv[u][5], xv[u][6], xv[u][7]);
+                dc_remove_stage<N, TPA, DcSum>(xv[u], t, A.dc_window, q, scan_part + u * NW, xs[u]);
             }
         }
         __syncthreads();
@@ -811,11 +816,23 @@ __device__ __forceinline__ void chain_body(const ChainArgs A, float2* bufA, floa
 // One launch over a frame: persistent workgroups (the grid is sized to the
 // resident capacity) stride over the A-scans; in M_STATS mode each
 // workgroup takes one contiguous chunk so that segment runs are long.
-// Lanczos (T = 16) holds 128 tap weights per lane: left to itself the
-// compiler spends AGPRs on the DC prologue and drops to one wave per SIMD;
-// two waves per SIMD with a few spills measured faster.
+// Occupancy targets: left to itself the compiler often lands a few
+// registers above 168 and so at two waves per SIMD where three fit without
+// spills; three is asked for wherever it needs no spills (every
+// throughput-mode kernel up to 4 taps, and DC removal of 8/16-bit samples).
+// Lanczos (T = 16) holds 128 tap weights per lane: two waves per SIMD with a
+// few spills measured faster than the one the compiler picks.  The FPN
+// statistics modes and the double-precision DC prologue keep the default.
+template <int T, int MODE, int IW>
+constexpr int chain_waves_per_eu() {
+    if (T == 16) return 2;
+    if ((MODE & M_STATS) || (IW == 4 && (MODE & M_DC))) return OPE_WAVES_PER_EU;
+    return 3;
+}
+
 template <int N, int T, int MODE, int IW, int AS = 1, int SCHED = -1>
-__global__ void __launch_bounds__(Geo<N>::WG) __attribute__((amdgpu_waves_per_eu(T == 16 ? 2 : OPE_WAVES_PER_EU)))
+__global__ void __launch_bounds__(Geo<N>::WG)
+__attribute__((amdgpu_waves_per_eu(chain_waves_per_eu<T, MODE, IW>())))
 chain_kernel(ChainArgs A) {
     constexpr int WG = Geo<N>::WG;
     using L = ChainLds<N, MODE, IW, AS, WG>;
