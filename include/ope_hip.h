@@ -210,8 +210,10 @@ OPE_HIP_API int ope_hip_op_min_variance_mean(ope_hip_ctx* ctx, const float* inpu
 /* ---- Launch autotuning.  No counterpart in the reference (its CUDA
  * backend launches fixed grids, cuda_backend.cu:486-743).  Times the chain's
  * launch variants for the context's current configuration (interpolation,
- * DC removal, input width): 1 or 2 A-scans per lane, the position of the
- * FFT's small-radix pass, 1 .. resident workgroups per CU; each variant
+ * DC removal, input width): 1 or 2 A-scans per lane (with dispersion off
+ * and no DC removal the two A-scans of a lane are real and share one complex
+ * FFT; that is also the default there), the position of the FFT's
+ * small-radix pass, 1 .. resident workgroups per CU; each variant
  * runs once untimed and 3 times timed on d_in -> d_out (n_ascans A-scans;
  * d_out is overwritten).  The fastest is used by later frames of that
  * configuration on this context.  With d_in and d_out NULL the context's own

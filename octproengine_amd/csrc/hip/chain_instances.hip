@@ -3,7 +3,7 @@
 //  AS = 1: taps {1, 2, 4, 16} x modes {plain, DC} x input widths {1, 2, 4}
 //          bytes x every radix placement, plus the FPN-statistics modes with
 //          the default placement;
-//  AS = 2: taps x modes {plain, DC} x widths, default placement.
+//  AS = 2: taps x modes {plain, DC, pairs} x widths, default placement.
 #include "chain_dispatch.h"
 
 #if !defined(OPE_CHAIN_N) || !defined(OPE_CHAIN_AS)
@@ -62,6 +62,9 @@ ChainLaunch pick_chain(int taps, int mode, int iw, int sched) {
         case M_DC: l.fn = pick_sched<N, M_DC, AS>(taps, iw, sched); break;
         case M_STATS:
             if constexpr (AS == 1) l.fn = pick_width<N, M_STATS, 1, LAST>(taps, iw);
+            break;
+        case M_PAIR:
+            if constexpr (AS == 2) l.fn = pick_width<N, M_PAIR, 2, LAST>(taps, iw);
             break;
         default:
             if constexpr (AS == 1) l.fn = pick_width<N, M_DC | M_STATS, 1, LAST>(taps, iw);

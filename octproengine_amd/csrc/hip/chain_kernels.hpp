@@ -436,6 +436,8 @@ __device__ __forceinline__ void stockham_inverse(cf (&v)[AS][8], float2* const (
 enum : int {
     M_DC = 1,      // rolling-average DC removal prologue
     M_STATS = 2,   // FPN determination: accumulate segment statistics, no output
+    M_PAIR = 4,    // no dispersion: the lane's two A-scans as the real and
+                   // imaginary part of one complex FFT (AS = 2)
 };
 
 // Rolling-average DC removal of one A-scan (cpu_kernels.tpp:56-82), staged
@@ -553,9 +555,14 @@ struct ChainLds {
     static_assert(WG % G::TPA == 0 && WG % 64 == 0, "workgroup must hold whole A-scans");
     static constexpr int SLOTS = (WG / G::TPA) * AS;        // A-scans per workgroup iteration
     static constexpr int NW = WG / 64;                      // wavefronts per workgroup
-    static constexpr bool X_ALIAS_B = (G::NPASS % 2) == 0;  // x staging can share buffer B
+    static constexpr bool PAIR = (MODE & M_PAIR) != 0;
+    static_assert(!PAIR || (AS == 2 && (MODE & (M_DC | M_STATS)) == 0), "pairs: two A-scans per lane, no DC/stats");
+    static constexpr int FFTS = PAIR ? SLOTS / 2 : SLOTS;               // complex FFTs per iteration
+    // x staging can share buffer B; not with pairs, whose buffer B also
+    // carries the spectrum exchange that splits the pair after the FFT
+    static constexpr bool X_ALIAS_B = (G::NPASS % 2) == 0 && !PAIR;
     using DcSum = typename std::conditional<(IW <= 2), int, double>::type;
-    static constexpr int BUF = SLOTS * G::PADN;                          // float2 per buffer
+    static constexpr int BUF = FFTS * G::PADN;                           // float2 per buffer
     static constexpr int X = X_ALIAS_B ? 1 : SLOTS * N;                 // floats
     static constexpr int SCAN = (MODE & M_DC) ? AS * NW : 1;            // DcSum
     static constexpr int MM = (MODE & M_STATS) ? 1 : 2 * AS * NW;       // floats
@@ -594,6 +601,7 @@ __device__ __forceinline__ void chain_body(const ChainArgs A, float2* bufA, floa
     constexpr bool DC = (MODE & M_DC) != 0;
     constexpr bool STATS = (MODE & M_STATS) != 0;
     constexpr bool X_ALIAS_B = L::X_ALIAS_B;
+    constexpr bool PAIR = L::PAIR;
     using DcSum = typename L::DcSum;
 
     // one A-scan per workgroup iteration: the A-scan index is uniform
@@ -604,8 +612,10 @@ __device__ __forceinline__ void chain_body(const ChainArgs A, float2* bufA, floa
     float* xs[AS];
 #pragma unroll
     for (int u = 0; u < AS; ++u) {
-        pa[u] = bufA + (slot * AS + u) * G::PADN;
-        pb[u] = bufB + (slot * AS + u) * G::PADN;
+        // pairs share one FFT (and its buffers) per slot
+        const int f = PAIR ? slot : slot * AS + u;
+        pa[u] = bufA + f * G::PADN;
+        pb[u] = bufB + f * G::PADN;
         xs[u] = X_ALIAS_B ? reinterpret_cast<float*>(pb[u]) : (xsep + (slot * AS + u) * N);
     }
 
@@ -625,7 +635,7 @@ __device__ __forceinline__ void chain_body(const ChainArgs A, float2* bufA, floa
         tb[e] = xs[0] + gmem(A.tap_base)[j];
 #pragma unroll
         for (int i = 0; i < T; ++i) tw8[e][i] = gmem(A.tap_w)[j * T + i];
-        if (flags & F_DISP) {
+        if (!PAIR && (flags & F_DISP)) {
             float2 p = gload2(A.phasor, j);
             ph[e] = {p.x, p.y};
         } else {
@@ -641,8 +651,9 @@ __device__ __forceinline__ void chain_body(const ChainArgs A, float2* bufA, floa
         for (int o = 0; o < 4; ++o) {
             const int k = t + o * TPA;
             if (flags & F_FPN) {
+                // pairs split into twice the spectrum (exact scaling)
                 float2 f = gload2(A.fpn, k);
-                fp[o] = {f.x, f.y};
+                fp[o] = PAIR ? cf{2.f * f.x, 2.f * f.y} : cf{f.x, f.y};
             } else {
                 fp[o] = {0.f, 0.f};
             }
@@ -714,8 +725,39 @@ __device__ __forceinline__ void chain_body(const ChainArgs A, float2* bufA, floa
         }
 
         // ---- inverse FFT: NP Stockham passes
-        stockham_inverse<N, AS, SCHED>(v, pa, pb, tw);
-        // after the last pass v[u][e] holds X[t + e*TPA]; e < 4 are the kept bins
+        if constexpr (PAIR) {
+            // z = y0 + i y1: Z = X0 + i X1 with X0, X1 the spectra of the two
+            // real sequences, so X0[k] = (Z[k] + conj Z[N-k]) / 2 and
+            // X1[k] = (Z[k] - conj Z[N-k]) / 2i.
+            cf z[1][8];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) z[0][e] = {v[0][e].x, v[1][e].x};
+            float2* const za[1] = {pa[0]};
+            float2* const zb[1] = {pb[0]};
+            stockham_inverse<N, 1, SCHED>(z, za, zb, tw);
+            // Z[t + e*TPA] to the buffer the last pass did not read (its
+            // readers passed the previous pass's barrier), then Z[N - k]
+            float2* zx = ((NP - 1) % 2 == 1) ? pb[0] : pa[0];
+            const int base_w = pad_slots(t);
+#pragma unroll
+            for (int e = 0; e < 8; ++e) zx[base_w + pad_slots(e * TPA)] = make_float2(z[0][e].x, z[0][e].y);
+            __syncthreads();
+            // N - (t + e*TPA) = (TPA - t) + (7 - e)*TPA for t > 0; lane 0's
+            // partners Z[(8 - e)*TPA mod N] are its own elements
+            const int base_r = pad_slots(t > 0 ? TPA - t : 1);
+#pragma unroll
+            for (int o = 0; o < 4; ++o) {
+                const float2 q = zx[base_r + pad_slots((7 - o) * TPA)];
+                const cf w = z[0][(8 - o) & 7];
+                const cf p = t > 0 ? cf{q.x, q.y} : w;
+                v[0][o] = {z[0][o].x + p.x, z[0][o].y - p.y};  // 2 X0[k]
+                v[1][o] = {z[0][o].y + p.y, p.x - z[0][o].x};  // 2 X1[k]
+            }
+        } else {
+            stockham_inverse<N, AS, SCHED>(v, pa, pb, tw);
+        }
+        // after the last pass v[u][e] holds X[t + e*TPA] (twice X for
+        // pairs); e < 4 are the kept bins
 
 #pragma unroll
         for (int u = 0; u < AS; ++u) {
@@ -755,13 +797,17 @@ __device__ __forceinline__ void chain_body(const ChainArgs A, float2* bufA, floa
                     m2[o] = fmaf(z.x, z.x, z.y * z.y);
                 }
                 float res[4];
+                // pairs: m2 = 4 |X|^2, i.e. log2 one higher by 2, sqrt twice
+                const float log_h = PAIR ? A.log_h + 2.0f : A.log_h;
+                const float ep_a = (PAIR && !(flags & F_LOG)) ? 0.5f * A.ep_a : A.ep_a;
+                const float ep_b = (PAIR && (flags & F_LOG)) ? fmaf(-2.0f, A.ep_a, A.ep_b) : A.ep_b;
                 if (flags & F_LOG) {
                     if (flags & F_AUTOMM) {
                         float val[4];
                         float mn = INFINITY, mx = -INFINITY;
 #pragma unroll
                         for (int o = 0; o < 4; ++o) {
-                            val[o] = A.log_c * (__log2f(m2[o]) - A.log_h);
+                            val[o] = A.log_c * (__log2f(m2[o]) - log_h);
                             mn = fminf(mn, val[o]);
                             mx = fmaxf(mx, val[o]);
                         }
@@ -772,11 +818,11 @@ __device__ __forceinline__ void chain_body(const ChainArgs A, float2* bufA, floa
                         for (int o = 0; o < 4; ++o) res[o] = A.coeff * ((val[o] - mn) / range + A.addend);
                     } else {
 #pragma unroll
-                        for (int o = 0; o < 4; ++o) res[o] = fmaf(A.ep_a, __log2f(m2[o]), A.ep_b);
+                        for (int o = 0; o < 4; ++o) res[o] = fmaf(ep_a, __log2f(m2[o]), ep_b);
                     }
                 } else {
 #pragma unroll
-                    for (int o = 0; o < 4; ++o) res[o] = fmaf(A.ep_a, __fsqrt_rn(m2[o]), A.ep_b);
+                    for (int o = 0; o < 4; ++o) res[o] = fmaf(ep_a, __fsqrt_rn(m2[o]), ep_b);
                 }
                 if (active) {
                     long long row = a;

@@ -402,10 +402,19 @@ ChainArgs base_args(ope_hip_ctx* c) {
     return a;
 }
 
-// Launches one chain variant: `as` A-scans per lane, small-radix pass at
-// `sched`, `per_cu` workgroups per CU (0: as many as are resident).
+// Without dispersion the A-scans are real: two A-scans per lane then share
+// one complex FFT (the real and the imaginary part) instead of taking one
+// each.  Pairs exist for the throughput mode without DC removal.
+bool pairable(const ChainArgs& a, int mode) { return !(a.flags & F_DISP) && (mode & (M_DC | M_STATS)) == 0; }
+
+int variant_mode(const ChainArgs& a, int mode, int as) { return (as == 2 && pairable(a, mode)) ? mode | M_PAIR : mode; }
+
+// Launches one chain variant: `as` A-scans per lane (paired into one FFT
+// where pairable), small-radix pass at `sched`, `per_cu` workgroups per CU
+// (0: as many as are resident).
 int launch_variant(ope_hip_ctx* c, const ChainArgs& a, int mode, int as, int sched, int per_cu_want,
                    long long work_ascans, hipStream_t s) {
+    mode = variant_mode(a, mode, as);
     const ChainLaunch l = pick_kernel(c->n, c->taps, mode, (int)input_bytes(c->p.input_type), as, sched);
     if (!l.fn) return fail(c, OPE_HIP_EUNSUP, "chain variant not built (mode %d, %d A-scans per lane)", mode, as);
     int per_cu = 0;
@@ -426,7 +435,7 @@ int launch_chain(ope_hip_ctx* c, const ChainArgs& a, bool stats, long long work_
         for (const auto& t : c->tuning)
             if (t.taps == c->taps && t.mode == mode && t.iw == iw)
                 return launch_variant(c, a, mode, t.as, t.sched, t.per_cu, work_ascans, s);
-    return launch_variant(c, a, mode, 1, -1, 0, work_ascans, s);
+    return launch_variant(c, a, mode, pairable(a, mode) ? 2 : 1, -1, 0, work_ascans, s);
 }
 
 int zero_acc(ope_hip_ctx* c, hipStream_t s) {
@@ -1548,7 +1557,7 @@ int ope_hip_autotune(ope_hip_ctx* c, const void* d_in, float* d_out, int64_t n_a
         const int scheds = (as == 1 && (c->n >> (3 * (np - 1))) != 8) ? np : 1;
         for (int sc = 0; sc < scheds; ++sc) {
             const int sched = scheds == 1 ? -1 : sc;
-            const ChainLaunch l = pick_kernel(c->n, c->taps, mode, iw, as, sched);
+            const ChainLaunch l = pick_kernel(c->n, c->taps, variant_mode(a, mode, as), iw, as, sched);
             if (!l.fn) continue;
             int occ = 0;
             HIP_TRY(c, cached_occupancy(c, reinterpret_cast<const void*>(l.fn), l.wg, &occ));
