@@ -136,3 +136,36 @@ def test_pairs_fpn_profile_auto_min_max_background_flip():
     e = float(np.max(np.abs(auto - (db - mn) / (mx - mn))))
     print(f"auto min/max on pairs: max deviation from the rule {e:.2e}")
     assert e <= 1e-5
+
+
+
+def test_pairs_with_fpn_determination():
+    """One-shot FPN determination and subtraction without dispersion, paired
+    and pinned to one A-scan per FFT, against the oracle with the GPU's own
+    profile.  The profile removes the DC term, so the bar in dB is applied
+    within 40 dB of the reflector peak (as for the 2048 x 1024 x 16
+    workload), the linear bar on every bin."""
+    n, a = 2048, 256
+    p = db_params(_params(n, a, "cubic", fpn_removal=True, fpn_bscan_count=1))
+    raw = S.fringes(n, a, seed=17)
+    t = standard_tables(n)
+    outs, profs = [], []
+    for pinned in (False, True):
+        g = GpuChain(p, t.curve, t.window, t.phasor)
+        if pinned:
+            g.ctx.set_launch_variant(1, _passes(n) - 1, 0)
+        g.ctx.request_fpn()
+        outs.append(g.frame(raw))
+        profs.append(g.ctx.get_fpn_profile())
+        g.close()
+    assert profs[0].tobytes() == profs[1].tobytes()  # the statistics pass is the same kernel
+    st = O.ChainState()
+    st.fpn_profile = profs[0].astype(np.float32)
+    pr = _params(n, a, "cubic", fpn_removal=True, fpn_bscan_count=1000)
+    ref = O.process_frame(raw, db_params(pr), t.curve, t.window, t.phasor, st)
+    e_p, e_s = db_error(outs[0], ref, 40.0), db_error(outs[1], ref, 40.0)
+    el = db_lin_error(outs[0], ref)
+    print(f"FPN determination: dB err within 40 dB of peak pair={e_p:.2e} single={e_s:.2e}; "
+          f"within 60 dB pair={db_error(outs[0], ref):.2e} single={db_error(outs[1], ref):.2e}; "
+          f"pair peak-normalised linear {el:.2e}")
+    assert e_p <= DB_TOL and e_s <= DB_TOL and el <= LIN_TOL
