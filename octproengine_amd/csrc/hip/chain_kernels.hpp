@@ -11,12 +11,13 @@
 // Work layout: an A-scan of N samples is owned by TPA = N/8 lanes; lane t
 // holds the 8 complex values at positions t + e*TPA (e = 0..7) throughout the
 // FFT.  Every Stockham pass reads those positions from LDS, so all LDS reads
-// are unit-stride across lanes; pass outputs are written to a padded layout
-// (one float2 of padding per 8) that keeps the stride-8 writes of the first
-// pass conflict-free.  Because a lane always handles the same positions, the
+// are unit-stride across lanes; pass outputs are written to padded layouts
+// (pad_slots, pad_pass1) that keep the strided writes free of bank
+// conflicts.  Because a lane always handles the same positions, the
 // per-lane tables (taps, phasor, twiddles, FPN profile, background) are read
 // once per kernel launch and stay in VGPRs while the workgroup loops over
-// A-scans (persistent grid-stride loop).
+// A-scans (persistent grid-stride loop).  Without dispersion two real
+// A-scans share one complex FFT (M_PAIR).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -307,8 +308,6 @@ __device__ __forceinline__ void ascan_minmax(float& mn, float& mx, float* part_m
     }
 }
 
-// ------------------------------------------------------------------ kernel
-
 // ------------------------------------------------------------------ FFT
 
 // Radix schedule of the Stockham passes: every pass is radix 8 except, when
@@ -356,8 +355,9 @@ __device__ __forceinline__ void load_twiddles(const float2* table, int t, cf (&t
 
 // Unnormalised inverse FFT (sign +) of the AS sequences whose elements
 // t + e*N/8 lane t holds in v[u][e]; on return v[u][e] = X[t + e*N/8].
-// Radix-8 Stockham passes (the last one of radix 8, 4 or 2) through the
-// padded LDS buffers pa/pb (pass p writes pa for even p, pb for odd p).
+// Radix-8 Stockham passes (where N is not a power of 8, one of radix 4 or
+// 2 at the position FftPlan gives) through the padded LDS buffers pa/pb
+// (pass p writes pa for even p, pb for odd p).
 // Ends without a barrier: the last pass's reads of pa/pb may still be in
 // flight in other wavefronts.
 template <int N, int AS, int SCHED = -1>
