@@ -174,3 +174,19 @@ def test_processor_state_before_initialize(ope):
 def test_cpu_backend_is_not_part_of_this_library(ope):
     with pytest.raises(ope.BackendError):
         ope.Processor(ope.Backend.CPU)
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU: --gpus N > 1 (or a distributed launch)
+    exits with status 2 and says why, before touching a device."""
+    import subprocess
+    import sys
+    from pathlib import Path
+    root = Path(__file__).resolve().parents[1]
+    for args, env in ((["--gpus", "2"], {}), ([], {"WORLD_SIZE": "4"})):
+        import os
+        r = subprocess.run([sys.executable, str(root / "bench.py"), *args], capture_output=True, text=True,
+                           env={**os.environ, **env}, timeout=120)
+        assert r.returncode == 2
+        assert "one GPU" in r.stderr
+        assert r.stdout == ""
