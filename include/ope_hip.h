@@ -213,7 +213,10 @@ OPE_HIP_API int ope_hip_op_min_variance_mean(ope_hip_ctx* ctx, const float* inpu
  * DC removal, input width): 1 or 2 A-scans per lane (with dispersion off
  * and no DC removal the two A-scans of a lane are real and share one complex
  * FFT; that is also the default there), the position of the FFT's
- * small-radix pass, 1 .. resident workgroups per CU; each variant
+ * small-radix pass (position = number of passes: the swap schedule of
+ * N = 1024 and 2048, radices 8, 8, N/512, 8 with the exchange into the
+ * small-radix pass made by lane-half register swaps instead of LDS),
+ * 1 .. resident workgroups per CU; each variant
  * runs once untimed and 3 times timed on d_in -> d_out (n_ascans A-scans;
  * d_out is overwritten).  The fastest is used by later frames of that
  * configuration on this context.  With d_in and d_out NULL the context's own
@@ -225,9 +228,10 @@ OPE_HIP_API int ope_hip_autotune(ope_hip_ctx* ctx, const void* d_in, float* d_ou
 
 /* Pins the launch variant of the context's current configuration instead
  * of tuning it: A-scans per lane (1 or 2), small-radix pass position
- * (0 .. passes-1; the last pass is the default; other positions exist for
- * one A-scan per lane only: OPE_HIP_EUNSUP), workgroups per CU (0: all
- * that are resident). */
+ * (0 .. passes-1; the last pass is the default; `passes` selects the swap
+ * schedule, built for N = 1024 and 2048; other positions and the swap
+ * schedule exist for one A-scan per lane only: OPE_HIP_EUNSUP), workgroups
+ * per CU (0: all that are resident). */
 OPE_HIP_API int ope_hip_set_launch_variant(ope_hip_ctx* ctx, int ascans_per_lane, int small_radix_pass,
                                            int workgroups_per_cu);
 

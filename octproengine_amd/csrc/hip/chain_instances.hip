@@ -1,8 +1,9 @@
 // Chain-kernel instantiations of one signal length and one number of
 // A-scans per lane, OPE_CHAIN_N and OPE_CHAIN_AS (set on the compiler line).
 //  AS = 1: taps {1, 2, 4, 16} x modes {plain, DC} x input widths {1, 2, 4}
-//          bytes x every radix placement, plus the FPN-statistics modes with
-//          the default placement;
+//          bytes x every radix placement (and, for N = 1024 and 2048, the
+//          swap schedule), plus the FPN-statistics modes with the default
+//          placement;
 //  AS = 2: taps x modes {plain, DC, pairs} x widths, default placement.
 #include "chain_dispatch.h"
 
@@ -37,6 +38,11 @@ template <int N, int MODE, int AS>
 KernelFn pick_sched(int taps, int iw, int sched) {
     constexpr int NP = FftPlan<N, -1>::NP;
     constexpr int LAST = NP - 1;
+    // NP: the swap schedule (exchange into the small-radix pass in registers)
+    if (sched == NP) {
+        if constexpr (AS == 1 && swap_sched<N, NP>()) return pick_width<N, MODE, AS, NP>(taps, iw);
+        return nullptr;
+    }
     if constexpr (AS == 1 && FftPlan<N, -1>::SMALL != 8) {
         switch (sched) {
             case 0: return pick_width<N, MODE, AS, 0>(taps, iw);

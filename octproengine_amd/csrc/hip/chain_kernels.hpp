@@ -432,6 +432,173 @@ __device__ __forceinline__ void stockham_inverse(cf (&v)[AS][8], float2* const (
     }
 }
 
+// ------------------------------------------------------------------ FFT with a register exchange
+//
+// For N = 1024 and 2048 (radices 8, 8, S, 8 with S = N/512) the exchange
+// into the small-radix pass can stay inside each wavefront.  Four-step form,
+// with n = TPA d0 + r0, r0 = (TPA/8) d1 + r1, r1 = 8 d2 + r2 and
+// k = k0 + 8 k1 + 64 k2 + 64 S K3:
+//   pass 0: DFT-8 over d0 (registers, lane t = r0), twiddle w_N^(r0 k0);
+//   exchange 1 through LDS: registers d1; lane bits 0-2 hold k0, the top
+//     in-wavefront lane bits (5, and 4 for S = 4) hold d2, the others r2;
+//   pass 1: DFT-8 over d1, twiddle w_TPA^(r1 k1);
+//   exchange 2 inside the wavefront: v_permlane32_swap transposes register
+//     bit 2 with lane bit 5 and (S = 4) v_permlane16_swap register bit 1
+//     with lane bit 4, so that d2 is in the registers and k1's top bits in
+//     those lanes (one VALU instruction per dword pair, no LDS, no barrier);
+//   pass 2: DFT-S over d2, twiddle w_(8S)^(r2 k2);
+//   exchange 3 through LDS: registers r2, lane t = k0 + 8 k1 + 64 k2;
+//   pass 3: DFT-8 over r2: v[e] = X[t + e*TPA], the layout stockham_inverse
+//     leaves, so everything after the FFT is shared.
+// The LDS addresses are sums of one constant per index bit (lane bits give
+// the lane's base, register bits a compile-time offset); the constants make
+// every 32-lane half of each 8-byte access hit 32 distinct bank pairs.
+// One exchange fewer through LDS than stockham_inverse (whose writes cost
+// about twice its reads), 8 or 16 swap instructions more per lane.
+
+// the swap schedule is launch variant "small-radix pass = NPASS"
+template <int N, int SCHED>
+constexpr bool swap_sched() {
+    return (N == 1024 || N == 2048) && SCHED == Geo<N>::NPASS;
+}
+
+template <int N>
+struct SwapFft {
+    static_assert(N == 1024 || N == 2048, "swap schedule: N = 1024 or 2048");
+    static constexpr int TPA = N / 8;
+    static constexpr int S = N / 512;   // radix of pass 2
+    static constexpr int RQ = 8 / S;    // pass-2 butterflies per lane
+    static constexpr int E3 = N == 2048 ? 264 : 136;   // exchange-3 stride of r2
+    // lane fields of the exchange-1 reader (and of the exchange-3 writer,
+    // whose d2 lanes then hold k1's top bits)
+    __device__ static int r2_of(int t) { return N == 2048 ? ((t >> 3) & 1) | ((t >> 5) & 6) : ((t >> 3) & 3) | ((t >> 4) & 4); }
+    __device__ static int d2_of(int t) { return N == 2048 ? (t >> 4) & 3 : (t >> 5) & 1; }
+    __device__ static int k1hi_of(int t) { return N == 2048 ? (t >> 3) & 6 : (t >> 3) & 4; }
+    // exchange 1: address of (k0, d1, d2, r2) = r2 + 8 d2 + skew(k0) + c(d1)
+    static constexpr int skew(int k0) {
+        return N == 2048 ? 34 * (k0 & 1) + 68 * ((k0 >> 1) & 1) + 144 * (k0 >> 2)
+                         : 36 * (k0 & 1) + 72 * ((k0 >> 1) & 1) + 144 * (k0 >> 2);
+    }
+    static constexpr int c_d1(int d1) { return N == 2048 ? 288 * d1 : 16 * (d1 & 1) + 284 * (d1 >> 1); }
+    // writer lane t = r0: d1 = r0 / (TPA/8), r1 = r0 % (TPA/8)
+    __device__ static int e1_write_base(int t) {
+        return N == 2048 ? (t & 31) + 288 * (t >> 5) : (t & 31) + 284 * (t >> 5);
+    }
+    __device__ static int e1_read_base(int t) { return r2_of(t) + 8 * d2_of(t) + skew(t & 7); }
+    // exchange 3: address of (k0, k1, k2, r2) = k0 + 8 k1 + 64 k2 + E3 r2;
+    // the writer's register e holds k1 % RQ = e % RQ and k2 = e / RQ
+    __device__ static int e3_write_base(int t) { return (t & 7) + 8 * k1hi_of(t) + E3 * r2_of(t); }
+    static constexpr int e3_write_off(int e) { return 8 * (e % RQ) + 64 * (e / RQ); }
+};
+
+// Per-lane twiddles of the swap schedule: tw[0][e] = w_N^(t e) (pass 0, lane
+// t = r0), tw[1][e] = w_TPA^(r1 e) (pass 1), tw[2][e] = w_(8S)^(r2 (e / RQ))
+// (pass 2); table = exp(+2 pi i k / N).
+template <int N>
+__device__ __forceinline__ void load_twiddles_swap(const float2* table, int t, cf (&tw)[Geo<N>::NPASS][8]) {
+    using F = SwapFft<N>;
+    const int r1 = 8 * F::d2_of(t) + F::r2_of(t);
+    const int r2 = F::r2_of(t);
+#pragma unroll
+    for (int e = 1; e < 8; ++e) {
+        float2 w = gload2(table, (t * e) & (N - 1));
+        tw[0][e] = {w.x, w.y};
+        w = gload2(table, (8 * r1 * e) & (N - 1));
+        tw[1][e] = {w.x, w.y};
+        w = gload2(table, (64 * r2 * (e / F::RQ)) & (N - 1));
+        tw[2][e] = {w.x, w.y};
+    }
+}
+
+// v_permlane32_swap / v_permlane16_swap on the two dwords of a complex pair:
+// a (register bit 0) keeps lanes whose bit 5 (4) is 0 and gains b's, b the
+// rest: the register bit and the lane bit trade places.
+__device__ __forceinline__ void swap_lane_bit5(cf& a, cf& b) {
+    const auto x = __builtin_amdgcn_permlane32_swap(__float_as_uint(a.x), __float_as_uint(b.x), false, false);
+    const auto y = __builtin_amdgcn_permlane32_swap(__float_as_uint(a.y), __float_as_uint(b.y), false, false);
+    a = {__uint_as_float(x[0]), __uint_as_float(y[0])};
+    b = {__uint_as_float(x[1]), __uint_as_float(y[1])};
+}
+__device__ __forceinline__ void swap_lane_bit4(cf& a, cf& b) {
+    const auto x = __builtin_amdgcn_permlane16_swap(__float_as_uint(a.x), __float_as_uint(b.x), false, false);
+    const auto y = __builtin_amdgcn_permlane16_swap(__float_as_uint(a.y), __float_as_uint(b.y), false, false);
+    a = {__uint_as_float(x[0]), __uint_as_float(y[0])};
+    b = {__uint_as_float(x[1]), __uint_as_float(y[1])};
+}
+
+// Unnormalised inverse FFT of the AS sequences whose elements t + e*N/8
+// lane t holds in v[u][e], by the swap schedule; on return v[u][e] =
+// X[t + e*N/8] as after stockham_inverse.  Exchange 1 goes through pa,
+// exchange 3 through pb; ends without a barrier (other wavefronts may still
+// read pb).  Every lane of each wavefront must be active.
+template <int N, int AS>
+__device__ __forceinline__ void fft_swap(cf (&v)[AS][8], float2* const (&pa)[AS], float2* const (&pb)[AS],
+                                         const cf (&tw)[Geo<N>::NPASS][8]) {
+    using F = SwapFft<N>;
+    const int t = threadIdx.x % F::TPA;
+    // pass 0 and exchange 1
+    const int w1 = F::e1_write_base(t);
+#pragma unroll
+    for (int u = 0; u < AS; ++u) {
+        idft8(v[u]);
+#pragma unroll
+        for (int e = 1; e < 8; ++e) v[u][e] = cmul(v[u][e], tw[0][e]);
+#pragma unroll
+        for (int e = 0; e < 8; ++e) pa[u][w1 + F::skew(e)] = make_float2(v[u][e].x, v[u][e].y);
+    }
+    __syncthreads();
+    const int r1 = F::e1_read_base(t);
+#pragma unroll
+    for (int u = 0; u < AS; ++u) {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const float2 w = pa[u][r1 + F::c_d1(e)];
+            v[u][e] = {w.x, w.y};
+        }
+    }
+    // pass 1, exchange 2 in registers, pass 2
+#pragma unroll
+    for (int u = 0; u < AS; ++u) {
+        idft8(v[u]);
+#pragma unroll
+        for (int e = 1; e < 8; ++e) v[u][e] = cmul(v[u][e], tw[1][e]);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) swap_lane_bit5(v[u][e], v[u][e + 4]);
+        if constexpr (F::S == 4) {
+            swap_lane_bit4(v[u][0], v[u][2]);
+            swap_lane_bit4(v[u][1], v[u][3]);
+            swap_lane_bit4(v[u][4], v[u][6]);
+            swap_lane_bit4(v[u][5], v[u][7]);
+            idft4(v[u][0], v[u][2], v[u][4], v[u][6]);
+            idft4(v[u][1], v[u][3], v[u][5], v[u][7]);
+        } else {
+            idft2(v[u][0], v[u][4]);
+            idft2(v[u][1], v[u][5]);
+            idft2(v[u][2], v[u][6]);
+            idft2(v[u][3], v[u][7]);
+        }
+#pragma unroll
+        for (int e = F::RQ; e < 8; ++e) v[u][e] = cmul(v[u][e], tw[2][e]);
+    }
+    // exchange 3 and pass 3
+    const int w3 = F::e3_write_base(t);
+#pragma unroll
+    for (int u = 0; u < AS; ++u) {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) pb[u][w3 + F::e3_write_off(e)] = make_float2(v[u][e].x, v[u][e].y);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < AS; ++u) {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const float2 w = pb[u][t + F::E3 * e];
+            v[u][e] = {w.x, w.y};
+        }
+        idft8(v[u]);
+    }
+}
+
 // Compile-time modes of the chain kernel.
 enum : int {
     M_DC = 1,      // rolling-average DC removal prologue
@@ -549,7 +716,7 @@ __device__ __forceinline__ void dc_remove_stage(const float (&xv)[8], int t, int
 // and of the auto min/max.  The kernels declare them as separate
 // __shared__ arrays: offsets into them then fold into the LDS instructions'
 // immediate fields, which a single byte array carved at run time loses.
-template <int N, int MODE, int IW, int AS, int WG>
+template <int N, int MODE, int IW, int AS, int WG, int SCHED = -1>
 struct ChainLds {
     using G = Geo<N>;
     static_assert(WG % G::TPA == 0 && WG % 64 == 0, "workgroup must hold whole A-scans");
@@ -559,8 +726,9 @@ struct ChainLds {
     static_assert(!PAIR || (AS == 2 && (MODE & (M_DC | M_STATS)) == 0), "pairs: two A-scans per lane, no DC/stats");
     static constexpr int FFTS = PAIR ? SLOTS / 2 : SLOTS;               // complex FFTs per iteration
     // x staging can share buffer B; not with pairs, whose buffer B also
-    // carries the spectrum exchange that splits the pair after the FFT
-    static constexpr bool X_ALIAS_B = (G::NPASS % 2) == 0 && !PAIR;
+    // carries the spectrum exchange that splits the pair after the FFT, and
+    // not with the swap schedule, whose last reads are of buffer B
+    static constexpr bool X_ALIAS_B = (G::NPASS % 2) == 0 && !PAIR && !swap_sched<N, SCHED>();
     using DcSum = typename std::conditional<(IW <= 2), int, double>::type;
     static constexpr int BUF = FFTS * G::PADN;                           // float2 per buffer
     static constexpr int X = X_ALIAS_B ? 1 : SLOTS * N;                 // floats
@@ -590,11 +758,11 @@ struct ChainLds {
 // this with the same range.
 template <int N, int T, int MODE, int IW, int AS, int WG, int SCHED = -1>
 __device__ __forceinline__ void chain_body(const ChainArgs A, float2* bufA, float2* bufB, float* xsep,
-                                           typename ChainLds<N, MODE, IW, AS, WG>::DcSum* scan_part,
-                                           typename ChainLds<N, MODE, IW, AS, WG>::DcSum* qsep,
+                                           typename ChainLds<N, MODE, IW, AS, WG, SCHED>::DcSum* scan_part,
+                                           typename ChainLds<N, MODE, IW, AS, WG, SCHED>::DcSum* qsep,
                                            float* mm_part, long long a_begin, long long a_end, long long a_step) {
     using G = Geo<N>;
-    using L = ChainLds<N, MODE, IW, AS, WG>;
+    using L = ChainLds<N, MODE, IW, AS, WG, SCHED>;
     constexpr int TPA = G::TPA;
     constexpr int NP = G::NPASS;
     constexpr int NW = L::NW;
@@ -602,6 +770,7 @@ __device__ __forceinline__ void chain_body(const ChainArgs A, float2* bufA, floa
     constexpr bool STATS = (MODE & M_STATS) != 0;
     constexpr bool X_ALIAS_B = L::X_ALIAS_B;
     constexpr bool PAIR = L::PAIR;
+    constexpr bool SWAP = swap_sched<N, SCHED>();
     using DcSum = typename L::DcSum;
 
     // one A-scan per workgroup iteration: the A-scan index is uniform
@@ -643,7 +812,10 @@ __device__ __forceinline__ void chain_body(const ChainArgs A, float2* bufA, floa
         }
     }
     cf tw[NP][8];
-    load_twiddles<N, SCHED>(A.twiddle, t, tw);
+    if constexpr (SWAP)
+        load_twiddles_swap<N>(A.twiddle, t, tw);
+    else
+        load_twiddles<N, SCHED>(A.twiddle, t, tw);
     cf fp[4];
     float bgv[4];
     if constexpr (!STATS) {
@@ -734,10 +906,13 @@ __device__ __forceinline__ void chain_body(const ChainArgs A, float2* bufA, floa
             for (int e = 0; e < 8; ++e) z[0][e] = {v[0][e].x, v[1][e].x};
             float2* const za[1] = {pa[0]};
             float2* const zb[1] = {pb[0]};
-            stockham_inverse<N, 1, SCHED>(z, za, zb, tw);
+            if constexpr (SWAP)
+                fft_swap<N, 1>(z, za, zb, tw);
+            else
+                stockham_inverse<N, 1, SCHED>(z, za, zb, tw);
             // Z[t + e*TPA] to the buffer the last pass did not read (its
             // readers passed the previous pass's barrier), then Z[N - k]
-            float2* zx = ((NP - 1) % 2 == 1) ? pb[0] : pa[0];
+            float2* zx = (SWAP || (NP - 1) % 2 == 0) ? pa[0] : pb[0];
             const int base_w = pad_slots(t);
 #pragma unroll
             for (int e = 0; e < 8; ++e) zx[base_w + pad_slots(e * TPA)] = make_float2(z[0][e].x, z[0][e].y);
@@ -753,6 +928,8 @@ __device__ __forceinline__ void chain_body(const ChainArgs A, float2* bufA, floa
                 v[0][o] = {z[0][o].x + p.x, z[0][o].y - p.y};  // 2 X0[k]
                 v[1][o] = {z[0][o].y + p.y, p.x - z[0][o].x};  // 2 X1[k]
             }
+        } else if constexpr (SWAP) {
+            fft_swap<N, AS>(v, pa, pb, tw);
         } else {
             stockham_inverse<N, AS, SCHED>(v, pa, pb, tw);
         }
@@ -881,7 +1058,7 @@ __global__ void __launch_bounds__(Geo<N>::WG)
 __attribute__((amdgpu_waves_per_eu(chain_waves_per_eu<T, MODE, IW>())))
 chain_kernel(ChainArgs A) {
     constexpr int WG = Geo<N>::WG;
-    using L = ChainLds<N, MODE, IW, AS, WG>;
+    using L = ChainLds<N, MODE, IW, AS, WG, SCHED>;
     __shared__ float2 bufA[L::BUF];
     __shared__ float2 bufB[L::BUF];
     __shared__ float xsep[L::X];

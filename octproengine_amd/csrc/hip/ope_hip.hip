@@ -1508,8 +1508,9 @@ int ope_hip_set_launch_variant(ope_hip_ctx* c, int ascans_per_lane, int small_ra
         return fail(c, OPE_HIP_EINVAL, "A-scans per lane must be 1 or 2");
     if (workgroups_per_cu < 0) return fail(c, OPE_HIP_EINVAL, "workgroups per CU must be >= 0");
     const int np = fft_passes(c->n);
-    if (small_radix_pass < 0 || small_radix_pass >= np)
-        return fail(c, OPE_HIP_EINVAL, "small-radix pass must be in 0..%d", np - 1);
+    // np: the swap schedule (N = 1024, 2048; built where pick_kernel finds it)
+    if (small_radix_pass < 0 || small_radix_pass > np)
+        return fail(c, OPE_HIP_EINVAL, "small-radix pass must be in 0..%d", np);
     const int mode = c->p.dc_removal_enabled ? M_DC : 0;
     const int iw = (int)input_bytes(c->p.input_type);
     const int sched = small_radix_pass == np - 1 ? -1 : small_radix_pass;
@@ -1557,7 +1558,9 @@ int ope_hip_autotune(ope_hip_ctx* c, const void* d_in, float* d_out, int64_t n_a
     std::vector<Cand> cands;
     const int np = fft_passes(c->n);
     for (int as = 1; as <= 2; ++as) {
-        const int scheds = (as == 1 && (c->n >> (3 * (np - 1))) != 8) ? np : 1;
+        // every placement of the small-radix pass, and the swap schedule
+        // (sched = np) where it is built
+        const int scheds = (as == 1 && (c->n >> (3 * (np - 1))) != 8) ? np + 1 : 1;
         for (int sc = 0; sc < scheds; ++sc) {
             const int sched = scheds == 1 ? -1 : sc;
             const ChainLaunch l = pick_kernel(c->n, c->taps, variant_mode(a, mode, as), iw, as, sched);
