@@ -1,10 +1,10 @@
 // Chain-kernel instantiations of one signal length and one number of
 // A-scans per lane, OPE_CHAIN_N and OPE_CHAIN_AS (set on the compiler line).
 //  AS = 1: taps {1, 2, 4, 16} x modes {plain, DC} x input widths {1, 2, 4}
-//          bytes x every radix placement (and, for N = 1024 and 2048, the
-//          swap schedule), plus the FPN-statistics modes with the default
-//          placement;
-//  AS = 2: taps x modes {plain, DC, pairs} x widths, default placement.
+//          bytes x every radix placement, plus the FPN-statistics modes with
+//          the default placement;
+//  AS = 2: taps x modes {plain, DC, pairs} x widths, default placement;
+//  both:   for N = 1024 and 2048 also the swap schedule (sched = passes).
 #include "chain_dispatch.h"
 
 #if !defined(OPE_CHAIN_N) || !defined(OPE_CHAIN_AS)
@@ -40,7 +40,7 @@ KernelFn pick_sched(int taps, int iw, int sched) {
     constexpr int LAST = NP - 1;
     // NP: the swap schedule (exchange into the small-radix pass in registers)
     if (sched == NP) {
-        if constexpr (AS == 1 && swap_sched<N, NP>()) return pick_width<N, MODE, AS, NP>(taps, iw);
+        if constexpr (swap_sched<N, NP>()) return pick_width<N, MODE, AS, NP>(taps, iw);
         return nullptr;
     }
     if constexpr (AS == 1 && FftPlan<N, -1>::SMALL != 8) {
@@ -70,7 +70,14 @@ ChainLaunch pick_chain(int taps, int mode, int iw, int sched) {
             if constexpr (AS == 1) l.fn = pick_width<N, M_STATS, 1, LAST>(taps, iw);
             break;
         case M_PAIR:
-            if constexpr (AS == 2) l.fn = pick_width<N, M_PAIR, 2, LAST>(taps, iw);
+            if constexpr (AS == 2) {
+                constexpr int NP = LAST + 1;
+                if (sched == NP) {
+                    if constexpr (swap_sched<N, NP>()) l.fn = pick_width<N, M_PAIR, 2, NP>(taps, iw);
+                } else {
+                    l.fn = pick_width<N, M_PAIR, 2, LAST>(taps, iw);
+                }
+            }
             break;
         default:
             if constexpr (AS == 1) l.fn = pick_width<N, M_DC | M_STATS, 1, LAST>(taps, iw);

@@ -726,9 +726,14 @@ struct ChainLds {
     static_assert(!PAIR || (AS == 2 && (MODE & (M_DC | M_STATS)) == 0), "pairs: two A-scans per lane, no DC/stats");
     static constexpr int FFTS = PAIR ? SLOTS / 2 : SLOTS;               // complex FFTs per iteration
     // x staging can share buffer B; not with pairs, whose buffer B also
-    // carries the spectrum exchange that splits the pair after the FFT, and
-    // not with the swap schedule, whose last reads are of buffer B
-    static constexpr bool X_ALIAS_B = (G::NPASS % 2) == 0 && !PAIR && !swap_sched<N, SCHED>();
+    // carries the spectrum exchange that splits the pair after the FFT.  The
+    // swap schedule's last reads are of buffer B: with one A-scan per lane
+    // the staging gets its own array, with two (where three arrays would
+    // leave room for one workgroup per CU) it shares B behind a barrier at
+    // the top of each iteration (SWAP_TOP_BARRIER)
+    static constexpr bool SWAP = swap_sched<N, SCHED>();
+    static constexpr bool SWAP_TOP_BARRIER = SWAP && AS == 2 && !PAIR;
+    static constexpr bool X_ALIAS_B = ((G::NPASS % 2) == 0 && !PAIR && !SWAP) || SWAP_TOP_BARRIER;
     using DcSum = typename std::conditional<(IW <= 2), int, double>::type;
     static constexpr int BUF = FFTS * G::PADN;                           // float2 per buffer
     static constexpr int X = X_ALIAS_B ? 1 : SLOTS * N;                 // floats
@@ -854,6 +859,9 @@ __device__ __forceinline__ void chain_body(const ChainArgs A, float2* bufA, floa
     }
 
     for (long long a0 = a_begin; a0 < a_end; a0 += a_step) {
+        // the staging shares buffer B, which the previous A-scan's last FFT
+        // pass may still be reading in other wavefronts
+        if constexpr (L::SWAP_TOP_BARRIER) __syncthreads();
         // ---- stage converted samples in LDS
         float xv[AS][8];
 #pragma unroll

@@ -438,7 +438,10 @@ int launch_chain(ope_hip_ctx* c, const ChainArgs& a, bool stats, long long work_
         for (const auto& t : c->tuning)
             if (t.taps == c->taps && t.mode == mode && t.iw == iw)
                 return launch_variant(c, a, mode, t.as, t.sched, t.per_cu, work_ascans, s);
-    return launch_variant(c, a, mode, pairable(a, mode) ? 2 : 1, -1, 0, work_ascans, s);
+    // untuned: pairs where pairable; the swap schedule where it is built
+    // (N = 1024, 2048: faster in every configuration measured)
+    const int sched = (c->n == 1024 || c->n == 2048) ? fft_passes(c->n) : -1;
+    return launch_variant(c, a, mode, pairable(a, mode) ? 2 : 1, sched, 0, work_ascans, s);
 }
 
 int zero_acc(ope_hip_ctx* c, hipStream_t s) {
@@ -1515,8 +1518,9 @@ int ope_hip_set_launch_variant(ope_hip_ctx* c, int ascans_per_lane, int small_ra
     const int iw = (int)input_bytes(c->p.input_type);
     const int sched = small_radix_pass == np - 1 ? -1 : small_radix_pass;
     const ChainLaunch l = pick_kernel(c->n, c->taps, mode, iw, ascans_per_lane, sched);
-    // placements other than the last exist for one A-scan per lane only
-    if (!l.fn || (ascans_per_lane == 2 && sched != -1))
+    // placements other than the last (and the swap schedule) exist for one
+    // A-scan per lane only
+    if (!l.fn || (ascans_per_lane == 2 && sched != -1 && sched != np))
         return fail(c, OPE_HIP_EUNSUP, "launch variant not built");
     store_tuning(c, mode, iw, ascans_per_lane, sched, workgroups_per_cu);
     return OPE_HIP_OK;
@@ -1558,11 +1562,15 @@ int ope_hip_autotune(ope_hip_ctx* c, const void* d_in, float* d_out, int64_t n_a
     std::vector<Cand> cands;
     const int np = fft_passes(c->n);
     for (int as = 1; as <= 2; ++as) {
-        // every placement of the small-radix pass, and the swap schedule
-        // (sched = np) where it is built
-        const int scheds = (as == 1 && (c->n >> (3 * (np - 1))) != 8) ? np + 1 : 1;
-        for (int sc = 0; sc < scheds; ++sc) {
-            const int sched = scheds == 1 ? -1 : sc;
+        // every placement of the small-radix pass (one A-scan per lane), and
+        // the swap schedule (sched = np) where it is built
+        const bool small = (c->n >> (3 * (np - 1))) != 8;
+        std::vector<int> scheds;
+        if (as == 1 && small)
+            for (int sc = 0; sc < np - 1; ++sc) scheds.push_back(sc);
+        scheds.push_back(-1);
+        if (small) scheds.push_back(np);
+        for (const int sched : scheds) {
             const ChainLaunch l = pick_kernel(c->n, c->taps, variant_mode(a, mode, as), iw, as, sched);
             if (!l.fn) continue;
             int occ = 0;

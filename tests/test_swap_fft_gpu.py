@@ -33,14 +33,14 @@ def _params(n, a, method, **kw):
     return p
 
 
-def _both(p, raw, t, setup=None, wg_per_cu=0):
+def _both(p, raw, t, setup=None, wg_per_cu=0, ascans_per_lane=1):
     """(swap-schedule output, default-variant output) of the same frame."""
     out = []
     for sched in (PASSES, PASSES - 1):
         g = GpuChain(p, t.curve, t.window, t.phasor, dtype=raw.dtype)
         if setup:
             setup(g)
-        g.ctx.set_launch_variant(1, sched, wg_per_cu)
+        g.ctx.set_launch_variant(ascans_per_lane, sched, wg_per_cu)
         out.append(g.frame(raw))
         g.close()
     return out
@@ -141,20 +141,36 @@ def test_swap_every_occupancy(wg_per_cu):
     assert db_error(swap, ref) <= DB_TOL and db_lin_error(swap, ref) <= LIN_TOL
 
 
+@pytest.mark.parametrize("n", [1024, 2048])
+@pytest.mark.parametrize("case", ["dispersion", "pairs", "pairs-lanczos", "dc"])
+def test_swap_two_ascans_per_lane(n, case):
+    """Two A-scans per lane: with dispersion each takes its own FFT (the
+    staging shares buffer B behind a barrier at the top of each iteration);
+    without dispersion the two are the real and imaginary part of one."""
+    a = 67
+    kw = {"dispersion": case == "dispersion" or case == "dc"}
+    if case == "dc":
+        kw.update(dc_removal=True, dc_window=48)
+    p = db_params(_params(n, a, "lanczos" if case == "pairs-lanczos" else "cubic", **kw))
+    raw = S.fringes(n, a, seed=900 + n + len(case))
+    t = standard_tables(n)
+    swap, default = _both(p, raw, t, ascans_per_lane=2)
+    ref = O.process_frame(raw, p, t.curve, t.window, t.phasor)
+    e_s, e_d = db_error(swap, ref), db_error(default, ref)
+    print(f"N={n} {case}, 2 A-scans per lane: dB err near peak swap={e_s:.2e} default={e_d:.2e}")
+    assert np.isfinite(swap).all()
+    assert e_s <= DB_TOL and db_lin_error(swap, ref) <= LIN_TOL
+    assert e_d <= DB_TOL
+
+
 def test_swap_not_built_elsewhere():
     from octproengine_amd import _capi
-    for n, passes in ((256, 3), (4096, 4)):
+    for n, passes in ((256, 3), (512, 3), (4096, 4)):
         p = _params(n, 16, "cubic")
         t = standard_tables(n)
         g = GpuChain(p, t.curve, t.window, t.phasor)
-        with pytest.raises(_capi.HipError) as e:
-            g.ctx.set_launch_variant(1, passes, 0)
-        assert e.value.code == _capi.EUNSUP
+        for as_ in (1, 2):
+            with pytest.raises(_capi.HipError) as e:
+                g.ctx.set_launch_variant(as_, passes, 0)
+            assert e.value.code == _capi.EUNSUP
         g.close()
-    p = _params(2048, 16, "cubic")
-    t = standard_tables(2048)
-    g = GpuChain(p, t.curve, t.window, t.phasor)
-    with pytest.raises(_capi.HipError) as e:
-        g.ctx.set_launch_variant(2, PASSES, 0)  # one A-scan per lane only
-    assert e.value.code == _capi.EUNSUP
-    g.close()
