@@ -44,10 +44,18 @@ struct RaggedArgs {
 
 constexpr int kRaggedWG = 512;
 
+// FFT schedule of a ragged body: the swap schedule where it is built
+// (N = 1024, 2048), as in the chain's default launch; its separate staging
+// array is no larger than the one the 256- and 512-point bodies need anyway
+template <int N>
+constexpr int ragged_sched() {
+    return (N == 1024 || N == 2048) ? Geo<N>::NPASS : -1;
+}
+
 template <int MODE, int IW>
 struct RaggedLds {
     template <int N>
-    using L = ChainLds<N, MODE, IW, 1, kRaggedWG>;
+    using L = ChainLds<N, MODE, IW, 1, kRaggedWG, ragged_sched<N>()>;
     static constexpr int cmax(int a, int b) { return a > b ? a : b; }
     static constexpr int BUF = cmax(cmax(cmax(L<256>::BUF, L<512>::BUF), cmax(L<1024>::BUF, L<2048>::BUF)), L<4096>::BUF);
     static constexpr int X = cmax(cmax(cmax(L<256>::X, L<512>::X), cmax(L<1024>::X, L<2048>::X)), L<4096>::X);
@@ -61,10 +69,13 @@ struct RaggedLds {
 template <int N, int T, int MODE, int IW>
 __device__ __attribute__((noinline)) void ragged_task(const ChainArgs& A, float2* bufA, float2* bufB, float* xsep,
         typename ChainLds<N, MODE, IW, 1, kRaggedWG>::DcSum* scan_part, float* mm_part, long long a0, long long a1) {
+    constexpr int SCHED = ragged_sched<N>();
+    using L = ChainLds<N, MODE, IW, 1, kRaggedWG, SCHED>;
     // the ragged geometry keeps the DC prefix sums in buffer A (no separate array)
-    static_assert(!ChainLds<N, MODE, IW, 1, kRaggedWG>::Q_SEP, "ragged tasks have no separate prefix array");
-    chain_body<N, T, MODE, IW, 1, kRaggedWG>(A, bufA, bufB, xsep, scan_part, nullptr, mm_part, a0, a1,
-                                             ChainLds<N, MODE, IW, 1, kRaggedWG>::SLOTS);
+    static_assert(!L::Q_SEP, "ragged tasks have no separate prefix array");
+    static_assert(L::X <= RaggedLds<MODE, IW>::X, "staging array");
+    chain_body<N, T, MODE, IW, 1, kRaggedWG, SCHED>(A, bufA, bufB, xsep, scan_part, nullptr, mm_part, a0, a1,
+                                                    L::SLOTS);
 }
 
 template <int T, int MODE, int IW>

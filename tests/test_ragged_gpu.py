@@ -25,6 +25,8 @@ def make_chains(lengths, interp=O.CUBIC, dtype=np.uint16, dc=False):
                                     resampling=True, interpolation=interp, windowing=True, dispersion=True,
                                     dc_removal=dc, dc_window=32))
         t = standard_tables(n)
+        # the default launch runs the FFT schedule of the ragged body of
+        # each length (the swap schedule at N = 1024 and 2048)
         chains[n] = (GpuChain(p, t.curve, t.window, t.phasor, dtype=dtype), p, t)
     return chains
 

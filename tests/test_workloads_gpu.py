@@ -175,6 +175,8 @@ def test_ragged_512_requests_full_size():
     for n in lengths:
         p = _params(n, 64, 1, O.CUBIC, dispersion=True)
         t = standard_tables(n)
+        # the default launch runs the ragged body's FFT schedule of each
+        # length (the swap schedule at N = 1024 and 2048)
         chains[n] = (GpuChain(p, t.curve, t.window, t.phasor, dtype=np.float32), p, t)
     reqs = []
     for i in range(512):
