@@ -353,6 +353,14 @@ __device__ __forceinline__ void load_twiddles(const float2* table, int t, cf (&t
     }
 }
 
+// Work a caller runs inside an FFT once the last pass's LDS reads are
+// issued: after the FFT's last workgroup barrier, so every wavefront has
+// finished the FFT's first exchange and every LDS read that came before it.
+// The default does nothing.
+struct NoHook {
+    __device__ void operator()() const {}
+};
+
 // Unnormalised inverse FFT (sign +) of the AS sequences whose elements
 // t + e*N/8 lane t holds in v[u][e]; on return v[u][e] = X[t + e*N/8].
 // Radix-8 Stockham passes (where N is not a power of 8, one of radix 4 or
@@ -360,9 +368,9 @@ __device__ __forceinline__ void load_twiddles(const float2* table, int t, cf (&t
 // (pass p writes pa for even p, pb for odd p).
 // Ends without a barrier: the last pass's reads of pa/pb may still be in
 // flight in other wavefronts.
-template <int N, int AS, int SCHED = -1>
+template <int N, int AS, int SCHED = -1, typename Hook = NoHook>
 __device__ __forceinline__ void stockham_inverse(cf (&v)[AS][8], float2* const (&pa)[AS], float2* const (&pb)[AS],
-                                                 const cf (&tw)[Geo<N>::NPASS][8]) {
+                                                 const cf (&tw)[Geo<N>::NPASS][8], Hook&& after_last_reads = {}) {
     using G = Geo<N>;
     using P = FftPlan<N, SCHED>;
     constexpr int NP = G::NPASS;
@@ -388,6 +396,7 @@ __device__ __forceinline__ void stockham_inverse(cf (&v)[AS][8], float2* const (
                     v[u][e] = {w.x, w.y};
                 }
             }
+            if (p == NP - 1) after_last_reads();
 #pragma unroll
             for (int u = 0; u < AS; ++u) {
 #pragma unroll
@@ -531,9 +540,9 @@ __device__ __forceinline__ void swap_lane_bit4(cf& a, cf& b) {
 // X[t + e*N/8] as after stockham_inverse.  Exchange 1 goes through pa,
 // exchange 3 through pb; ends without a barrier (other wavefronts may still
 // read pb).  Every lane of each wavefront must be active.
-template <int N, int AS>
+template <int N, int AS, typename Hook = NoHook>
 __device__ __forceinline__ void fft_swap(cf (&v)[AS][8], float2* const (&pa)[AS], float2* const (&pb)[AS],
-                                         const cf (&tw)[Geo<N>::NPASS][8]) {
+                                         const cf (&tw)[Geo<N>::NPASS][8], Hook&& after_last_reads = {}) {
     using F = SwapFft<N>;
     const int t = threadIdx.x % F::TPA;
     // pass 0 and exchange 1
@@ -595,8 +604,10 @@ __device__ __forceinline__ void fft_swap(cf (&v)[AS][8], float2* const (&pa)[AS]
             const float2 w = pb[u][t + F::E3 * e];
             v[u][e] = {w.x, w.y};
         }
-        idft8(v[u]);
     }
+    after_last_reads();
+#pragma unroll
+    for (int u = 0; u < AS; ++u) idft8(v[u]);
 }
 
 // Compile-time modes of the chain kernel.
@@ -858,23 +869,49 @@ __device__ __forceinline__ void chain_body(const ChainArgs A, float2* bufA, floa
         if (a < a_end) load_raw<IW>(A.in, N, a, t, raw[u]);
     }
 
+    // Staging of A-scan group a: convert the raw words (loaded one group
+    // ahead) into the staging array, then prefetch group a + a_step.
+    // Pipelined (PIPE): group a0 + a_step is staged inside the FFT of group
+    // a0, once its last LDS reads are issued: the staging array is free by
+    // then (every wavefront has passed the barrier after the tap gather),
+    // and the stores overlap the last pass instead of preceding a barrier.
+    // Not with DC removal (its own prologue stages), pairs (the split
+    // exchange after the FFT) or a staging array that shares buffer B with
+    // the swap schedule's last reads.
+    constexpr bool PIPE = !DC && !PAIR && !L::SWAP_TOP_BARRIER;
+    auto stage = [&](long long a) {
+#pragma unroll
+        for (int u = 0; u < AS; ++u) {
+            float x8[8];
+            convert_raw<IW>(ieee, shift, raw[u], x8);
+            float4* dst = reinterpret_cast<float4*>(xs[u] + 8 * t);
+            dst[0] = make_float4(x8[0], x8[1], x8[2], x8[3]);
+            dst[1] = make_float4(x8[4], x8[5], x8[6], x8[7]);
+            const long long an = a + a_step + slot * AS + u;
+            if (an < a_end) load_raw<IW>(A.in, N, an, t, raw[u]);
+        }
+    };
+    if constexpr (PIPE) stage(a_begin);
+
     for (long long a0 = a_begin; a0 < a_end; a0 += a_step) {
         // the staging shares buffer B, which the previous A-scan's last FFT
         // pass may still be reading in other wavefronts
         if constexpr (L::SWAP_TOP_BARRIER) __syncthreads();
         // ---- stage converted samples in LDS
         float xv[AS][8];
+        if constexpr (!PIPE) {
 #pragma unroll
-        for (int u = 0; u < AS; ++u) {
-            convert_raw<IW>(ieee, shift, raw[u], xv[u]);
-            if constexpr (!DC) {
-                float4* dst = reinterpret_cast<float4*>(xs[u] + 8 * t);
-                dst[0] = make_float4(xv[u][0], xv[u][1], xv[u][2], xv[u][3]);
-                dst[1] = make_float4(xv[u][4], xv[u][5], xv[u][6], xv[u][7]);
+            for (int u = 0; u < AS; ++u) {
+                convert_raw<IW>(ieee, shift, raw[u], xv[u]);
+                if constexpr (!DC) {
+                    float4* dst = reinterpret_cast<float4*>(xs[u] + 8 * t);
+                    dst[0] = make_float4(xv[u][0], xv[u][1], xv[u][2], xv[u][3]);
+                    dst[1] = make_float4(xv[u][4], xv[u][5], xv[u][6], xv[u][7]);
+                }
+                // prefetch the next A-scan while this one is processed
+                const long long an = a0 + a_step + slot * AS + u;
+                if (an < a_end) load_raw<IW>(A.in, N, an, t, raw[u]);
             }
-            // prefetch the next A-scan while this one is processed
-            const long long an = a0 + a_step + slot * AS + u;
-            if (an < a_end) load_raw<IW>(A.in, N, an, t, raw[u]);
         }
 
         if constexpr (DC) {
@@ -936,10 +973,16 @@ __device__ __forceinline__ void chain_body(const ChainArgs A, float2* bufA, floa
                 v[0][o] = {z[0][o].x + p.x, z[0][o].y - p.y};  // 2 X0[k]
                 v[1][o] = {z[0][o].y + p.y, p.x - z[0][o].x};  // 2 X1[k]
             }
-        } else if constexpr (SWAP) {
-            fft_swap<N, AS>(v, pa, pb, tw);
         } else {
-            stockham_inverse<N, AS, SCHED>(v, pa, pb, tw);
+            auto next = [&]() {
+                if constexpr (PIPE) {
+                    if (a0 + a_step < a_end) stage(a0 + a_step);
+                }
+            };
+            if constexpr (SWAP)
+                fft_swap<N, AS>(v, pa, pb, tw, next);
+            else
+                stockham_inverse<N, AS, SCHED>(v, pa, pb, tw, next);
         }
         // after the last pass v[u][e] holds X[t + e*TPA] (twice X for
         // pairs); e < 4 are the kept bins
