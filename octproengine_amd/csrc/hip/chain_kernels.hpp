@@ -875,10 +875,11 @@ __device__ __forceinline__ void chain_body(const ChainArgs A, float2* bufA, floa
     // a0, once its last LDS reads are issued: the staging array is free by
     // then (every wavefront has passed the barrier after the tap gather),
     // and the stores overlap the last pass instead of preceding a barrier.
-    // Not with DC removal (its own prologue stages), pairs (the split
-    // exchange after the FFT) or a staging array that shares buffer B with
-    // the swap schedule's last reads.
-    constexpr bool PIPE = !DC && !PAIR && !L::SWAP_TOP_BARRIER;
+    // Not with DC removal (its own prologue stages) or a staging array that
+    // shares buffer B with the swap schedule's last reads.  (Pairs have a
+    // staging array of their own; their split exchange after the FFT uses
+    // the FFT buffers.)
+    constexpr bool PIPE = !DC && !L::SWAP_TOP_BARRIER;
     auto stage = [&](long long a) {
 #pragma unroll
         for (int u = 0; u < AS; ++u) {
@@ -942,6 +943,11 @@ __device__ __forceinline__ void chain_body(const ChainArgs A, float2* bufA, floa
         }
 
         // ---- inverse FFT: NP Stockham passes
+        auto next = [&]() {
+            if constexpr (PIPE) {
+                if (a0 + a_step < a_end) stage(a0 + a_step);
+            }
+        };
         if constexpr (PAIR) {
             // z = y0 + i y1: Z = X0 + i X1 with X0, X1 the spectra of the two
             // real sequences, so X0[k] = (Z[k] + conj Z[N-k]) / 2 and
@@ -952,9 +958,9 @@ __device__ __forceinline__ void chain_body(const ChainArgs A, float2* bufA, floa
             float2* const za[1] = {pa[0]};
             float2* const zb[1] = {pb[0]};
             if constexpr (SWAP)
-                fft_swap<N, 1>(z, za, zb, tw);
+                fft_swap<N, 1>(z, za, zb, tw, next);
             else
-                stockham_inverse<N, 1, SCHED>(z, za, zb, tw);
+                stockham_inverse<N, 1, SCHED>(z, za, zb, tw, next);
             // Z[t + e*TPA] to the buffer the last pass did not read (its
             // readers passed the previous pass's barrier), then Z[N - k]
             float2* zx = (SWAP || (NP - 1) % 2 == 0) ? pa[0] : pb[0];
@@ -974,11 +980,6 @@ __device__ __forceinline__ void chain_body(const ChainArgs A, float2* bufA, floa
                 v[1][o] = {z[0][o].y + p.y, p.x - z[0][o].x};  // 2 X1[k]
             }
         } else {
-            auto next = [&]() {
-                if constexpr (PIPE) {
-                    if (a0 + a_step < a_end) stage(a0 + a_step);
-                }
-            };
             if constexpr (SWAP)
                 fft_swap<N, AS>(v, pa, pb, tw, next);
             else
