@@ -875,19 +875,26 @@ __device__ __forceinline__ void chain_body(const ChainArgs A, float2* bufA, floa
     // a0, once its last LDS reads are issued: the staging array is free by
     // then (every wavefront has passed the barrier after the tap gather),
     // and the stores overlap the last pass instead of preceding a barrier.
-    // Not with DC removal (its own prologue stages) or a staging array that
-    // shares buffer B with the swap schedule's last reads.  (Pairs have a
-    // staging array of their own; their split exchange after the FFT uses
-    // the FFT buffers.)
-    constexpr bool PIPE = !DC && !L::SWAP_TOP_BARRIER;
+    // With DC removal the whole prologue (prefix sums, window means, their
+    // barriers) moves with it where every array it writes is free by then:
+    // the prefix sums in their own array and a staging array of its own.
+    // Not where the staging shares buffer B with the swap schedule's last
+    // reads.  (Pairs have a staging array of their own; their split exchange
+    // after the FFT uses the FFT buffers.)
+    constexpr bool PIPE = DC ? (L::Q_SEP && !X_ALIAS_B) : !L::SWAP_TOP_BARRIER;
     auto stage = [&](long long a) {
 #pragma unroll
         for (int u = 0; u < AS; ++u) {
             float x8[8];
             convert_raw<IW>(ieee, shift, raw[u], x8);
-            float4* dst = reinterpret_cast<float4*>(xs[u] + 8 * t);
-            dst[0] = make_float4(x8[0], x8[1], x8[2], x8[3]);
-            dst[1] = make_float4(x8[4], x8[5], x8[6], x8[7]);
+            if constexpr (DC) {
+                dc_remove_stage<N, TPA, DcSum>(x8, t, A.dc_window, qsep + (slot * AS + u) * N, scan_part + u * NW,
+                                               xs[u]);
+            } else {
+                float4* dst = reinterpret_cast<float4*>(xs[u] + 8 * t);
+                dst[0] = make_float4(x8[0], x8[1], x8[2], x8[3]);
+                dst[1] = make_float4(x8[4], x8[5], x8[6], x8[7]);
+            }
             const long long an = a + a_step + slot * AS + u;
             if (an < a_end) load_raw<IW>(A.in, N, an, t, raw[u]);
         }
@@ -915,7 +922,7 @@ __device__ __forceinline__ void chain_body(const ChainArgs A, float2* bufA, floa
             }
         }
 
-        if constexpr (DC) {
+        if constexpr (DC && !PIPE) {
             // Prefix sums aliasing buffer A: the previous A-scan's last pass
             // may still be reading A.  With TPA > 64 the first barrier inside
             // dc_remove_stage already orders those reads before the writes.
