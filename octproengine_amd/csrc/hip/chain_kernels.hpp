@@ -84,6 +84,23 @@ __device__ __forceinline__ void idft8(cf* v) {
 
 __host__ __device__ constexpr int ilog2(int n) { return n <= 1 ? 0 : 1 + ilog2(n / 2); }
 
+// Orders the LDS accesses of the lanes that own one A-scan (TPA of them).
+// Up to 64 lanes they are one wavefront, whose LDS instructions execute in
+// order, and every LDS array of that A-scan is private to it: a wavefront
+// barrier (compiler fences, no hardware barrier) then suffices and the
+// workgroup's other wavefronts are not held up.  Otherwise a workgroup
+// barrier.
+template <int TPA>
+__device__ __forceinline__ void ascan_sync() {
+    if constexpr (TPA <= 64) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    } else {
+        __syncthreads();
+    }
+}
+
 // Padded LDS slot of complex element i (one float2 of padding per 32
 // elements, one more per 128).  Every shift is >= 3, so the Stockham read
 // (t + e*TPA) and write ((b/ns)*ns*R + b%ns + m*ns) addresses of a lane
@@ -436,7 +453,7 @@ __device__ __forceinline__ void stockham_inverse(cf (&v)[AS][8], float2* const (
                     }
                 }
             }
-            __syncthreads();
+            ascan_sync<TPA>();
         }
     }
 }
@@ -672,7 +689,7 @@ __device__ __forceinline__ void dc_remove_stage(const float (&xv)[8], int t, int
     const S excl = incl - s;
 #pragma unroll
     for (int i = 0; i < 8; ++i) Q[8 * t + i] = excl + loc[i];
-    __syncthreads();
+    ascan_sync<TPA>();
 
     // samples t + i*TPA.  With 1 <= W <= TPA - 2, i = 1..6 are always
     // inside the A-scan, i = 0 is inside iff t >= W, i = 7 iff
@@ -929,14 +946,14 @@ __device__ __forceinline__ void chain_body(const ChainArgs A, float2* bufA, floa
             // Prefix sums aliasing buffer A: the previous A-scan's last pass
             // may still be reading A.  With TPA > 64 the first barrier inside
             // dc_remove_stage already orders those reads before the writes.
-            if constexpr (!L::Q_SEP && TPA <= 64) __syncthreads();
+            if constexpr (!L::Q_SEP && TPA <= 64) ascan_sync<TPA>();
 #pragma unroll
             for (int u = 0; u < AS; ++u) {
                 DcSum* q = L::Q_SEP ? qsep + (slot * AS + u) * N : reinterpret_cast<DcSum*>(pa[u]);
                 dc_remove_stage<N, TPA, DcSum>(xv[u], t, A.dc_window, q, scan_part + u * NW, xs[u]);
             }
         }
-        __syncthreads();
+        ascan_sync<TPA>();
         // Raised wave priority until the tap gather is done: the waves that
         // leave the barrier get their LDS reads out ahead of the other
         // workgroups' arithmetic (2048-point cubic frames ~2 % faster,
@@ -985,7 +1002,7 @@ __device__ __forceinline__ void chain_body(const ChainArgs A, float2* bufA, floa
             const int base_w = pad_slots(t);
 #pragma unroll
             for (int e = 0; e < 8; ++e) zx[base_w + pad_slots(e * TPA)] = make_float2(z[0][e].x, z[0][e].y);
-            __syncthreads();
+            ascan_sync<TPA>();
             // N - (t + e*TPA) = (TPA - t) + (7 - e)*TPA for t > 0; lane 0's
             // partners Z[(8 - e)*TPA mod N] are its own elements
             const int base_r = pad_slots(t > 0 ? TPA - t : 1);
