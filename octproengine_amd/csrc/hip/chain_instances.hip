@@ -59,8 +59,8 @@ KernelFn pick_sched(int taps, int iw, int sched) {
 template <int N, int AS>
 ChainLaunch pick_chain(int taps, int mode, int iw, int sched) {
     ChainLaunch l;
-    l.wg = Geo<N>::WG;
-    l.slots = Geo<N>::APW * AS;
+    l.wg = chain_wg<N>();
+    l.slots = chain_wg<N>() / Geo<N>::TPA * AS;
     constexpr int LAST = FftPlan<N, -1>::NP - 1;
     l.fn = nullptr;
     switch (mode) {

@@ -124,7 +124,7 @@ template <int N>
 struct Geo {
     static constexpr int LOG2N = ilog2(N);
     static constexpr int TPA = N / 8;                       // lanes per A-scan
-    static constexpr int WG = TPA >= 256 ? TPA : 256;       // workgroup size of chain_kernel
+    static constexpr int WG = TPA >= 256 ? TPA : 256;       // workgroup of the autofocus and operation kernels
     static constexpr int APW = WG / TPA;                    // A-scans per chain_kernel workgroup
     static constexpr int NPASS = (LOG2N + 2) / 3;           // Stockham passes
     static constexpr int LAST_R = N >> (3 * (NPASS - 1));   // radix of the one non-radix-8 pass
@@ -138,6 +138,14 @@ struct Geo {
     static_assert((1 << LOG2N) == N, "N must be a power of two");
     static_assert(N >= 256 && N <= 4096, "N outside 256..4096");
 };
+
+// Workgroup of chain_kernel: one A-scan from N = 1024 on (a workgroup
+// barrier then holds up only the wavefronts that share the A-scan's LDS
+// arrays), 256 lanes below (where the syncs are per wavefront anyway).
+template <int N>
+constexpr int chain_wg() {
+    return Geo<N>::TPA >= 128 ? Geo<N>::TPA : 256;
+}
 
 
 // ------------------------------------------------------------------ arguments
@@ -772,12 +780,13 @@ struct ChainLds {
     static constexpr int MM = (MODE & M_STATS) ? 1 : 2 * AS * NW;       // floats
     // DC prefix sums in an array of their own (else they alias buffer A and
     // wait for the previous A-scan's last pass): chain_kernel's geometry
-    // with one A-scan per lane, where three workgroups still fit in the
-    // 160 KiB of LDS of a CU
+    // with one A-scan per lane, where three workgroups (six of 128 lanes,
+    // the same three wavefronts per SIMD) still fit in the 160 KiB of LDS
+    // of a CU
     static constexpr int BASE_BYTES = 2 * BUF * 8 + X * 4 + SCAN * (int)sizeof(DcSum) + MM * 4;
     static constexpr int Q_BYTES = SLOTS * N * (int)sizeof(DcSum);
-    static constexpr bool Q_SEP = (MODE & M_DC) != 0 && AS == 1 && WG == G::WG &&
-                                  3 * (BASE_BYTES + Q_BYTES) <= 160 * 1024;
+    static constexpr bool Q_SEP = (MODE & M_DC) != 0 && AS == 1 && WG == chain_wg<N>() &&
+                                  (WG >= 256 ? 3 : 768 / WG) * (BASE_BYTES + Q_BYTES) <= 160 * 1024;
     static constexpr int Q = Q_SEP ? SLOTS * N : 1;                      // DcSum
 };
 
@@ -1141,10 +1150,10 @@ constexpr int chain_waves_per_eu() {
 }
 
 template <int N, int T, int MODE, int IW, int AS = 1, int SCHED = -1>
-__global__ void __launch_bounds__(Geo<N>::WG)
+__global__ void __launch_bounds__(chain_wg<N>())
 __attribute__((amdgpu_waves_per_eu(chain_waves_per_eu<T, MODE, IW>())))
 chain_kernel(ChainArgs A) {
-    constexpr int WG = Geo<N>::WG;
+    constexpr int WG = chain_wg<N>();
     using L = ChainLds<N, MODE, IW, AS, WG, SCHED>;
     __shared__ float2 bufA[L::BUF];
     __shared__ float2 bufB[L::BUF];
