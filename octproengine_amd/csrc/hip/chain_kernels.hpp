@@ -737,11 +737,17 @@ __device__ __forceinline__ void dc_remove_stage(const float (&xv)[8], int t, int
         const S sum = Q[end] - (start > 0 ? Q[start - 1] : S(0));
         const int cnt = end - start + 1;
         const S x = (S)xs[n];
-        // x - sum/cnt with a single rounding of the result
-        if constexpr (sizeof(S) == 4)
-            xs[n] = (float)(__mul24(x, cnt) - sum) / (float)cnt;  // numerator exact
-        else
+        // x - sum/cnt with a single rounding of the result; windows inside
+        // the A-scan as in the band (reciprocal of 2W), so that wavefronts
+        // without a clipped window skip the division
+        if constexpr (sizeof(S) == 4) {
+            if (inside)
+                xs[n] = (float)(__mul24(x, 2 * W) - sum) * (1.0f / (float)(2 * W));
+            else
+                xs[n] = (float)(__mul24(x, cnt) - sum) / (float)cnt;  // numerator exact
+        } else {
             xs[n] = (float)(x - sum / (double)cnt);
+        }
     }
 }
 
