@@ -651,23 +651,78 @@ enum : int {
 // x[start .. end], start = max(n - (W-1), 0), end = min(n + W, N-1) for
 // W >= 1, and start = 0, end = n for W = 0.  Lane t brings samples
 // n = 8t .. 8t+7 in xv.  Window sums are differences of exact inclusive
-// prefix sums Q[n] = x[0] + ... + x[n] (N entries of S in LDS: int for
-// 8/16-bit samples, double for 32-bit ones); `part` holds one S per
-// wavefront.
-//   The prefix sums are built in the lane's own 8 consecutive samples; the
-// means are then taken with lane t on samples t + i*TPA, so that every LDS
-// access of that step is lane-consecutive (no bank conflicts) and the
-// clipped windows at both ends fall on one sample per lane (i = 0 and
+// prefix sums Q[n] = x[0] + ... + x[n] (int for 8/16-bit samples, double
+// for 32-bit ones), built in the lane's own 8 consecutive samples; `part`
+// holds one S per wavefront.
+//
+// Windows of 1 .. DC_BAND samples over int sums (the banded path): lane t
+// keeps its own samples.  The prefix sums go to Q in blocks of 8 at a
+// stride of 9 ints (block b at 9b: the stores of a wavefront, and its loads
+// of 8 consecutive sums from any offset, hit 64 distinct banks), with
+// DC_BAND/8 blocks of zeros (Q[-DC_BAND .. -1]) before the A-scan's and as
+// many blocks of the total (Q[N .. N+DC_BAND-1]) after it, so that clipped
+// windows need no clamping.  Lane t loads Q[8t+i+W] and Q[8t+i-W]
+// (i = 0..7) with offsets fixed per W mod 8.  The numerator x*cnt - sum is
+// an exact integer below 2^24; windows inside the A-scan (cnt = 2W) scale it
+// by the reciprocal of 2W, the lanes whose windows clip divide by cnt
+// through the reciprocal and one correction step (within one ulp).
+// Q_OVER_X: the banded path's prefix array ends where the general path's
+// N-entry array Q ends and overlaps the end of xs, so the samples are
+// stored only after a barrier behind the loads.
+//
+// The general path (W = 0, W > DC_BAND, double sums): samples to xs, the
+// prefix sums to Q[0 .. N-1], then the means with lane t on samples
+// t + i*TPA, so that every LDS access of that step is lane-consecutive and
+// the clipped windows at both ends fall on one sample per lane (i = 0 and
 // i = 7) as long as W <= TPA - 2.  For i = 1..6 the windows lie inside the
-// A-scan (cnt = 2W): the exact integer numerator x*2W - sum is scaled by the
-// reciprocal of 2W (within one ulp); the end samples divide exactly.
-// Contains two workgroup barriers (one for TPA <= 64); every earlier read of
-// Q, part and xs must be ordered before the call.
+// A-scan (cnt = 2W): the exact integer numerator x*2W - sum is scaled by
+// the reciprocal of 2W (within one ulp); the end samples divide exactly.
+//
+// Q holds N entries of S; with S = int and !Q_OVER_X also dc_prefix_ints
+// ints (the banded path's).  Contains two workgroup barriers (one for
+// TPA <= 64), three on the banded path with Q_OVER_X; every earlier read
+// of Q, part and xs must be ordered before the call.
+constexpr int DC_BAND = 64;
+template <int TPA>
+constexpr int dc_prefix_ints() { return 9 * (TPA + DC_BAND / 4); }
+
+// The banded path's samples x - (hi[.] - lo[.]) / 2W for W mod 8 = B, to
+// dst (the lane's 8 staging floats), in two halves of 4: the block pad sits
+// after the first 8-B sums from hi and after the first B from lo (none for
+// B = 0).  The numerator x*2W - sum is exact in float (below 2^24).
+template <int B>
+__device__ __forceinline__ void dc_band_means(const int* hi, const int* lo, const float (&xv)[8], float w2, float inv,
+                                              float4* dst) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        float r[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int i = 4 * h + k;
+            const float sum = (float)(hi[i + ((B != 0 && i >= 8 - B) ? 1 : 0)] - lo[i + ((B != 0 && i >= B) ? 1 : 0)]);
+            r[k] = fmaf(xv[i], w2, -sum) * inv;
+        }
+        dst[h] = make_float4(r[0], r[1], r[2], r[3]);
+    }
+}
+
+// the window the stages use: wider windows cover the whole A-scan alike;
+// the host rejects W < 0
+template <int N>
+__device__ __forceinline__ int dc_window(int window) { return window > N ? N : (window < 0 ? 0 : window); }
+
 template <int N, int TPA, typename S>
-__device__ __forceinline__ void dc_remove_stage(const float (&xv)[8], int t, int window, S* Q, S* part, float* xs) {
-    // wider windows cover the whole A-scan alike; the host rejects W < 0
-    const int W = window > N ? N : (window < 0 ? 0 : window);
-    {
+constexpr bool dc_banded() { return sizeof(S) == 4 && TPA >= 128 && dc_prefix_ints<TPA>() <= 2 * N; }
+
+// First half: the prefix sums (the samples too on the general path), and
+// the barrier behind them.
+template <int N, int TPA, typename S, bool BANDED>
+__device__ __forceinline__ void dc_prefix_stage(const float (&xv)[8], int t, int window, S* Q, S* part, float* xs) {
+    const int W = dc_window<N>(window);
+    constexpr bool BAND = BANDED && dc_banded<N, TPA, S>();
+    // without the banded path the samples go out first (fewer registers
+    // held through the scan)
+    if constexpr (!BAND) {
         float4* dst = reinterpret_cast<float4*>(xs + 8 * t);
         dst[0] = make_float4(xv[0], xv[1], xv[2], xv[3]);
         dst[1] = make_float4(xv[4], xv[5], xv[6], xv[7]);
@@ -695,9 +750,92 @@ __device__ __forceinline__ void dc_remove_stage(const float (&xv)[8], int t, int
         incl += off;
     }
     const S excl = incl - s;
+
+    constexpr int GB = DC_BAND / 8;  // guard blocks at each end
+    if constexpr (BAND) {
+        if (W >= 1 && W <= DC_BAND) {
+            int* const qp = reinterpret_cast<int*>(Q);
+            // the A-scan's total: its last lane's inclusive sum
+            int total;
+            if constexpr (TPA >= 64)
+                total = __builtin_amdgcn_readlane(incl, 63);  // read by the last wavefront only
+            else
+                total = __shfl(incl, ((int)threadIdx.x & 63) | (TPA - 1), 64);
+#pragma unroll
+            for (int i = 0; i < 8; ++i) qp[9 * (t + GB) + i] = excl + loc[i];
+            if (t < GB) {
+#pragma unroll
+                for (int i = 0; i < 8; ++i) qp[9 * t + i] = 0;
+            }
+            if (t >= TPA - GB) {
+#pragma unroll
+                for (int i = 0; i < 8; ++i) qp[9 * (t + 2 * GB) + i] = total;
+            }
+            ascan_sync<TPA>();
+            return;
+        }
+    }
+
+    if constexpr (BAND) {
+        float4* dst = reinterpret_cast<float4*>(xs + 8 * t);
+        dst[0] = make_float4(xv[0], xv[1], xv[2], xv[3]);
+        dst[1] = make_float4(xv[4], xv[5], xv[6], xv[7]);
+    }
 #pragma unroll
     for (int i = 0; i < 8; ++i) Q[8 * t + i] = excl + loc[i];
     ascan_sync<TPA>();
+}
+
+// Second half: the window means into xs.  The same xv, window and Q as
+// the first half; any code may run between the two that leaves Q, part and
+// xs alone.
+template <int N, int TPA, typename S, bool BANDED>
+__device__ __forceinline__ void dc_means_stage(const float (&xv)[8], int t, int window, S* Q, float* xs) {
+    const int W = dc_window<N>(window);
+    constexpr int GB = DC_BAND / 8;
+    if constexpr (BANDED && dc_banded<N, TPA, S>()) {
+        if (W >= 1 && W <= DC_BAND) {
+            int* const qp = reinterpret_cast<int*>(Q);
+            // Q[8t + W] and Q[8t - W] in the padded layout
+            const int* hi = qp + 9 * (t + GB) + W + (W >> 3);
+            const int* lo = qp + 9 * t + (DC_BAND - W) + ((DC_BAND - W) >> 3);
+            // every sample as if its window were inside the A-scan
+            const float w2 = (float)(2 * W);
+            const float inv = 1.0f / w2;
+            float4* dst = reinterpret_cast<float4*>(xs + 8 * t);
+            switch (W & 7) {
+                case 0: dc_band_means<0>(hi, lo, xv, w2, inv, dst); break;
+                case 1: dc_band_means<1>(hi, lo, xv, w2, inv, dst); break;
+                case 2: dc_band_means<2>(hi, lo, xv, w2, inv, dst); break;
+                case 3: dc_band_means<3>(hi, lo, xv, w2, inv, dst); break;
+                case 4: dc_band_means<4>(hi, lo, xv, w2, inv, dst); break;
+                case 5: dc_band_means<5>(hi, lo, xv, w2, inv, dst); break;
+                case 6: dc_band_means<6>(hi, lo, xv, w2, inv, dst); break;
+                default: dc_band_means<7>(hi, lo, xv, w2, inv, dst); break;
+            }
+            // then the 2W-1 clipped ones again, one per lane: n = t (n <= W-2,
+            // start clipped) in the first wavefront and n = N - TPA + t
+            // (n >= N-W, end clipped) in the last, the same wavefronts as
+            // those samples' owners above, so that these stores land after
+            // theirs.  x[n] = Q[n] - Q[n-1]; the guard blocks stand in for
+            // Q[-1] = 0 and Q[N-1 .. ] = total
+            const int n = t < 64 ? t : N - TPA + t;
+            if ((t < 64 && n <= W - 2) || (t >= TPA - 64 && n >= N - W)) {
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                auto qat = [&](int m) {  // Q[m], -DC_BAND <= m < N + DC_BAND
+                    const int j = m + DC_BAND;
+                    return qp[9 * (j >> 3) + (j & 7)];
+                };
+                const int x = qat(n) - qat(n - 1);
+                const int sum = qat(n + W) - qat(n - W);
+                const int start = n >= W - 1 ? n - (W - 1) : 0;
+                const int end = n + W < N - 1 ? n + W : N - 1;
+                const int cnt = end - start + 1;
+                xs[n] = (float)(__mul24(x, cnt) - sum) / (float)cnt;  // numerator exact
+            }
+            return;
+        }
+    }
 
     // samples t + i*TPA.  With 1 <= W <= TPA - 2, i = 1..6 are always
     // inside the A-scan, i = 0 is inside iff t >= W, i = 7 iff
@@ -751,6 +889,12 @@ __device__ __forceinline__ void dc_remove_stage(const float (&xv)[8], int t, int
     }
 }
 
+template <int N, int TPA, typename S, bool BANDED = false>
+__device__ __forceinline__ void dc_remove_stage(const float (&xv)[8], int t, int window, S* Q, S* part, float* xs) {
+    dc_prefix_stage<N, TPA, S, BANDED>(xv, t, window, Q, part, xs);
+    dc_means_stage<N, TPA, S, BANDED>(xv, t, window, Q, xs);
+}
+
 #ifndef OPE_WAVES_PER_EU
 #define OPE_WAVES_PER_EU 1
 #endif
@@ -781,19 +925,15 @@ struct ChainLds {
     static constexpr bool X_ALIAS_B = ((G::NPASS % 2) == 0 && !PAIR && !SWAP) || SWAP_TOP_BARRIER;
     using DcSum = typename std::conditional<(IW <= 2), int, double>::type;
     static constexpr int BUF = FFTS * G::PADN;                           // float2 per buffer
-    static constexpr int X = X_ALIAS_B ? 1 : SLOTS * N;                 // floats
     static constexpr int SCAN = (MODE & M_DC) ? AS * NW : 1;            // DcSum
     static constexpr int MM = (MODE & M_STATS) ? 1 : 2 * AS * NW;       // floats
-    // DC prefix sums in an array of their own (else they alias buffer A and
-    // wait for the previous A-scan's last pass): chain_kernel's geometry
-    // with one A-scan per lane, where three workgroups (six of 128 lanes,
-    // the same three wavefronts per SIMD) still fit in the 160 KiB of LDS
-    // of a CU
-    static constexpr int BASE_BYTES = 2 * BUF * 8 + X * 4 + SCAN * (int)sizeof(DcSum) + MM * 4;
-    static constexpr int Q_BYTES = SLOTS * N * (int)sizeof(DcSum);
-    static constexpr bool Q_SEP = (MODE & M_DC) != 0 && AS == 1 && WG == chain_wg<N>() &&
-                                  (WG >= 256 ? 3 : 768 / WG) * (BASE_BYTES + Q_BYTES) <= 160 * 1024;
-    static constexpr int Q = Q_SEP ? SLOTS * N : 1;                      // DcSum
+    static constexpr int X = X_ALIAS_B ? 1 : SLOTS * N;                 // floats
+    // DC removal staged inside the FFT (one A-scan per lane, a staging array
+    // of its own): the prefix sums then go to buffer A, which every
+    // wavefront has finished reading once the last pass's reads (of buffer
+    // B: the swap schedule's, and the Stockham schedule's with an odd number
+    // of passes, the only one with a staging array of its own) are issued
+    static constexpr bool DC_PIPE = (MODE & M_DC) != 0 && AS == 1 && !X_ALIAS_B;
 };
 
 // The chain over A-scans a_begin + k*a_step ... (< a_end) of A, each
@@ -810,7 +950,6 @@ struct ChainLds {
 template <int N, int T, int MODE, int IW, int AS, int WG, int SCHED = -1>
 __device__ __forceinline__ void chain_body(const ChainArgs A, float2* bufA, float2* bufB, float* xsep,
                                            typename ChainLds<N, MODE, IW, AS, WG, SCHED>::DcSum* scan_part,
-                                           typename ChainLds<N, MODE, IW, AS, WG, SCHED>::DcSum* qsep,
                                            float* mm_part, long long a_begin, long long a_end, long long a_step) {
     using G = Geo<N>;
     using L = ChainLds<N, MODE, IW, AS, WG, SCHED>;
@@ -916,15 +1055,35 @@ __device__ __forceinline__ void chain_body(const ChainArgs A, float2* bufA, floa
     // Not where the staging shares buffer B with the swap schedule's last
     // reads.  (Pairs have a staging array of their own; their split exchange
     // after the FFT uses the FFT buffers.)
-    constexpr bool PIPE = DC ? (L::Q_SEP && !X_ALIAS_B) : !L::SWAP_TOP_BARRIER;
-    auto stage = [&](long long a) {
+    // The banded DC path (windows of 1 .. DC_BAND samples) needs more
+    // registers than the kernels with resampling taps have to spare at three
+    // wavefronts per SIMD: only the kernels without taps take it.  Staged
+    // inside the FFT, it splits: the prefix sums go in there, the window
+    // means after the output stores, where the FFT's registers are free (the
+    // raw words of the staged group stay in registers in between).  The
+    // general path stages whole inside the FFT.
+    constexpr bool BANDED = T == 1;
+    constexpr bool SPLIT = DC && BANDED && dc_banded<N, TPA, DcSum>();
+    constexpr bool PIPE = DC ? L::DC_PIPE : !L::SWAP_TOP_BARRIER;
+    auto stage_prefix = [&]() {
 #pragma unroll
         for (int u = 0; u < AS; ++u) {
             float x8[8];
             convert_raw<IW>(ieee, shift, raw[u], x8);
-            if constexpr (DC) {
-                dc_remove_stage<N, TPA, DcSum>(x8, t, A.dc_window, qsep + (slot * AS + u) * N, scan_part + u * NW,
-                                               xs[u]);
+            dc_prefix_stage<N, TPA, DcSum, BANDED>(x8, t, A.dc_window, reinterpret_cast<DcSum*>(pa[u]),
+                                                   scan_part + u * NW, xs[u]);
+        }
+    };
+    auto stage_rest = [&](long long a) {
+#pragma unroll
+        for (int u = 0; u < AS; ++u) {
+            float x8[8];
+            convert_raw<IW>(ieee, shift, raw[u], x8);
+            if constexpr (SPLIT) {
+                dc_means_stage<N, TPA, DcSum, BANDED>(x8, t, A.dc_window, reinterpret_cast<DcSum*>(pa[u]), xs[u]);
+            } else if constexpr (DC) {
+                dc_remove_stage<N, TPA, DcSum, BANDED>(x8, t, A.dc_window, reinterpret_cast<DcSum*>(pa[u]),
+                                                       scan_part + u * NW, xs[u]);
             } else {
                 float4* dst = reinterpret_cast<float4*>(xs[u] + 8 * t);
                 dst[0] = make_float4(x8[0], x8[1], x8[2], x8[3]);
@@ -934,7 +1093,10 @@ __device__ __forceinline__ void chain_body(const ChainArgs A, float2* bufA, floa
             if (an < a_end) load_raw<IW>(A.in, N, an, t, raw[u]);
         }
     };
-    if constexpr (PIPE) stage(a_begin);
+    if constexpr (PIPE) {
+        if constexpr (SPLIT) stage_prefix();
+        stage_rest(a_begin);
+    }
 
     for (long long a0 = a_begin; a0 < a_end; a0 += a_step) {
         // the staging shares buffer B, which the previous A-scan's last FFT
@@ -961,11 +1123,11 @@ __device__ __forceinline__ void chain_body(const ChainArgs A, float2* bufA, floa
             // Prefix sums aliasing buffer A: the previous A-scan's last pass
             // may still be reading A.  With TPA > 64 the first barrier inside
             // dc_remove_stage already orders those reads before the writes.
-            if constexpr (!L::Q_SEP && TPA <= 64) ascan_sync<TPA>();
+            if constexpr (TPA <= 64) ascan_sync<TPA>();
 #pragma unroll
             for (int u = 0; u < AS; ++u) {
-                DcSum* q = L::Q_SEP ? qsep + (slot * AS + u) * N : reinterpret_cast<DcSum*>(pa[u]);
-                dc_remove_stage<N, TPA, DcSum>(xv[u], t, A.dc_window, q, scan_part + u * NW, xs[u]);
+                dc_remove_stage<N, TPA, DcSum, BANDED>(xv[u], t, A.dc_window, reinterpret_cast<DcSum*>(pa[u]),
+                                                       scan_part + u * NW, xs[u]);
             }
         }
         ascan_sync<TPA>();
@@ -995,7 +1157,12 @@ __device__ __forceinline__ void chain_body(const ChainArgs A, float2* bufA, floa
         // ---- inverse FFT: NP Stockham passes
         auto next = [&]() {
             if constexpr (PIPE) {
-                if (a0 + a_step < a_end) stage(a0 + a_step);
+                if (a0 + a_step < a_end) {
+                    if constexpr (SPLIT)
+                        stage_prefix();
+                    else
+                        stage_rest(a0 + a_step);
+                }
             }
         };
         if constexpr (PAIR) {
@@ -1120,6 +1287,9 @@ __device__ __forceinline__ void chain_body(const ChainArgs A, float2* bufA, floa
                 }
             }
         }
+        if constexpr (PIPE && SPLIT) {
+            if (a0 + a_step < a_end) stage_rest(a0 + a_step);
+        }
     }
 
     if constexpr (STATS) {
@@ -1165,7 +1335,6 @@ chain_kernel(ChainArgs A) {
     __shared__ float2 bufB[L::BUF];
     __shared__ float xsep[L::X];
     __shared__ typename L::DcSum scan_part[L::SCAN];
-    __shared__ typename L::DcSum qsep[L::Q];
     __shared__ float mm_part[L::MM];
     constexpr int SLOTS = L::SLOTS;
     long long a_begin, a_step, a_end;
@@ -1182,7 +1351,7 @@ chain_kernel(ChainArgs A) {
         a_end = A.n_ascans;
         a_step = (long long)gridDim.x * SLOTS;
     }
-    chain_body<N, T, MODE, IW, AS, WG, SCHED>(A, bufA, bufB, xsep, scan_part, qsep, mm_part, a_begin, a_end, a_step);
+    chain_body<N, T, MODE, IW, AS, WG, SCHED>(A, bufA, bufB, xsep, scan_part, mm_part, a_begin, a_end, a_step);
 }
 
 }  // namespace ope_hip

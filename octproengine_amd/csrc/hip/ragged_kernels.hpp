@@ -71,10 +71,8 @@ __device__ __attribute__((noinline)) void ragged_task(const ChainArgs& A, float2
         typename ChainLds<N, MODE, IW, 1, kRaggedWG>::DcSum* scan_part, float* mm_part, long long a0, long long a1) {
     constexpr int SCHED = ragged_sched<N>();
     using L = ChainLds<N, MODE, IW, 1, kRaggedWG, SCHED>;
-    // the ragged geometry keeps the DC prefix sums in buffer A (no separate array)
-    static_assert(!L::Q_SEP, "ragged tasks have no separate prefix array");
     static_assert(L::X <= RaggedLds<MODE, IW>::X, "staging array");
-    chain_body<N, T, MODE, IW, 1, kRaggedWG, SCHED>(A, bufA, bufB, xsep, scan_part, nullptr, mm_part, a0, a1,
+    chain_body<N, T, MODE, IW, 1, kRaggedWG, SCHED>(A, bufA, bufB, xsep, scan_part, mm_part, a0, a1,
                                                     L::SLOTS);
 }
 
