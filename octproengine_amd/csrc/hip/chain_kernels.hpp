@@ -799,6 +799,28 @@ __device__ __forceinline__ void dc_means_stage(const float (&xv)[8], int t, int 
             // Q[8t + W] and Q[8t - W] in the padded layout
             const int* hi = qp + 9 * (t + GB) + W + (W >> 3);
             const int* lo = qp + 9 * t + (DC_BAND - W) + ((DC_BAND - W) >> 3);
+            // The 2W-1 clipped samples are redone one per lane: n = t
+            // (n <= W-2, start clipped) in the first wavefront and
+            // n = N - TPA + t (n >= N-W, end clipped) in the last, the same
+            // wavefronts as those samples' owners, whose stores below these
+            // follow.  x[n] = Q[n] - Q[n-1]; the guard blocks stand in for
+            // Q[-1] = 0 and Q[N-1 ..] = total.  Computed first, so that its
+            // loads and division overlap the band's loads.
+            const int n = t < 64 ? t : N - TPA + t;
+            const bool fix = (t < 64 && n <= W - 2) || (t >= TPA - 64 && n >= N - W);
+            float fixed = 0.f;
+            if (fix) {
+                auto qat = [&](int m) {  // Q[m], -DC_BAND <= m < N + DC_BAND
+                    const int j = m + DC_BAND;
+                    return qp[9 * (j >> 3) + (j & 7)];
+                };
+                const int x = qat(n) - qat(n - 1);
+                const int sum = qat(n + W) - qat(n - W);
+                const int start = n >= W - 1 ? n - (W - 1) : 0;
+                const int end = n + W < N - 1 ? n + W : N - 1;
+                const int cnt = end - start + 1;
+                fixed = (float)(__mul24(x, cnt) - sum) / (float)cnt;  // numerator exact
+            }
             // every sample as if its window were inside the A-scan
             const float w2 = (float)(2 * W);
             const float inv = 1.0f / w2;
@@ -813,25 +835,9 @@ __device__ __forceinline__ void dc_means_stage(const float (&xv)[8], int t, int 
                 case 6: dc_band_means<6>(hi, lo, xv, w2, inv, dst); break;
                 default: dc_band_means<7>(hi, lo, xv, w2, inv, dst); break;
             }
-            // then the 2W-1 clipped ones again, one per lane: n = t (n <= W-2,
-            // start clipped) in the first wavefront and n = N - TPA + t
-            // (n >= N-W, end clipped) in the last, the same wavefronts as
-            // those samples' owners above, so that these stores land after
-            // theirs.  x[n] = Q[n] - Q[n-1]; the guard blocks stand in for
-            // Q[-1] = 0 and Q[N-1 .. ] = total
-            const int n = t < 64 ? t : N - TPA + t;
-            if ((t < 64 && n <= W - 2) || (t >= TPA - 64 && n >= N - W)) {
+            if (fix) {
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                auto qat = [&](int m) {  // Q[m], -DC_BAND <= m < N + DC_BAND
-                    const int j = m + DC_BAND;
-                    return qp[9 * (j >> 3) + (j & 7)];
-                };
-                const int x = qat(n) - qat(n - 1);
-                const int sum = qat(n + W) - qat(n - W);
-                const int start = n >= W - 1 ? n - (W - 1) : 0;
-                const int end = n + W < N - 1 ? n + W : N - 1;
-                const int cnt = end - start + 1;
-                xs[n] = (float)(__mul24(x, cnt) - sum) / (float)cnt;  // numerator exact
+                xs[n] = fixed;
             }
             return;
         }
