@@ -655,20 +655,18 @@ enum : int {
 // for 32-bit ones), built in the lane's own 8 consecutive samples; `part`
 // holds one S per wavefront.
 //
-// Windows of 1 .. DC_BAND samples over int sums (the banded path): lane t
-// keeps its own samples.  The prefix sums go to Q in blocks of 8 at a
-// stride of 9 ints (block b at 9b: the stores of a wavefront, and its loads
-// of 8 consecutive sums from any offset, hit 64 distinct banks), with
-// DC_BAND/8 blocks of zeros (Q[-DC_BAND .. -1]) before the A-scan's and as
-// many blocks of the total (Q[N .. N+DC_BAND-1]) after it, so that clipped
-// windows need no clamping.  Lane t loads Q[8t+i+W] and Q[8t+i-W]
-// (i = 0..7) with offsets fixed per W mod 8.  The numerator x*cnt - sum is
-// an exact integer below 2^24; windows inside the A-scan (cnt = 2W) scale it
-// by the reciprocal of 2W, the lanes whose windows clip divide by cnt
-// through the reciprocal and one correction step (within one ulp).
-// Q_OVER_X: the banded path's prefix array ends where the general path's
-// N-entry array Q ends and overlaps the end of xs, so the samples are
-// stored only after a barrier behind the loads.
+// The banded path (BANDED, int sums, TPA >= 128, windows of 1 .. DC_BAND
+// samples): lane t keeps its own samples.  The prefix sums go to Q in
+// blocks of 8 at a stride of 9 ints (block b at 9b: the stores of a
+// wavefront, and its loads of 8 consecutive sums from any offset, hit 64
+// distinct banks), with DC_BAND/8 blocks of zeros (Q[-DC_BAND .. -1])
+// before the A-scan's and as many blocks of the total (Q[N .. N+DC_BAND-1])
+// after it, so that clipped windows need no clamping.  Lane t loads
+// Q[8t+i+W] and Q[8t+i-W] (i = 0..7) with offsets fixed per W mod 8 and
+// takes every sample as an interior one (the exact numerator x*2W - sum
+// times the reciprocal of 2W); the 2W-1 samples whose windows clip are then
+// redone one per lane (the exact numerator x*cnt - sum times the reciprocal
+// of cnt with one correction step, within one ulp).
 //
 // The general path (W = 0, W > DC_BAND, double sums): samples to xs, the
 // prefix sums to Q[0 .. N-1], then the means with lane t on samples
@@ -678,10 +676,10 @@ enum : int {
 // A-scan (cnt = 2W): the exact integer numerator x*2W - sum is scaled by
 // the reciprocal of 2W (within one ulp); the end samples divide exactly.
 //
-// Q holds N entries of S; with S = int and !Q_OVER_X also dc_prefix_ints
-// ints (the banded path's).  Contains two workgroup barriers (one for
-// TPA <= 64), three on the banded path with Q_OVER_X; every earlier read
-// of Q, part and xs must be ordered before the call.
+// Q holds N entries of S, and dc_prefix_ints ints where the banded path can
+// run.  The prefix half contains two workgroup barriers (one for
+// TPA <= 64), the means half none; every earlier read of Q, part and xs
+// must be ordered before the prefix half.
 constexpr int DC_BAND = 64;
 template <int TPA>
 constexpr int dc_prefix_ints() { return 9 * (TPA + DC_BAND / 4); }
@@ -818,8 +816,13 @@ __device__ __forceinline__ void dc_means_stage(const float (&xv)[8], int t, int 
                 const int sum = qat(n + W) - qat(n - W);
                 const int start = n >= W - 1 ? n - (W - 1) : 0;
                 const int end = n + W < N - 1 ? n + W : N - 1;
-                const int cnt = end - start + 1;
-                fixed = (float)(__mul24(x, cnt) - sum) / (float)cnt;  // numerator exact
+                const float c = (float)(end - start + 1);
+                // x - sum/cnt from the exact numerator: reciprocal and one
+                // correction step (within one ulp)
+                const float num = (float)(__mul24(x, end - start + 1) - sum);
+                const float rc = __builtin_amdgcn_rcpf(c);
+                const float q0 = num * rc;
+                fixed = fmaf(fmaf(-q0, c, num), rc, q0);
             }
             // every sample as if its window were inside the A-scan
             const float w2 = (float)(2 * W);
