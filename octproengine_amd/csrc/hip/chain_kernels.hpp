@@ -1060,9 +1060,9 @@ __device__ __forceinline__ void chain_body(const ChainArgs A, float2* bufA, floa
     // and the stores overlap the last pass instead of preceding a barrier.
     // With DC removal the whole prologue (prefix sums, window means, their
     // barriers) moves with it where every array it writes is free by then:
-    // the prefix sums in their own array and a staging array of its own.
-    // Not where the staging shares buffer B with the swap schedule's last
-    // reads.  (Pairs have a staging array of their own; their split exchange
+    // a staging array of its own, and the prefix sums in buffer A (read for
+    // the last time before the last pass's barrier).  Not where the staging
+    // shares buffer B with the last reads.  (Pairs have a staging array of their own; their split exchange
     // after the FFT uses the FFT buffers.)
     // The banded DC path (windows of 1 .. DC_BAND samples) needs more
     // registers than the kernels with resampling taps have to spare at three
