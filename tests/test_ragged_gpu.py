@@ -19,10 +19,12 @@ LENGTHS = (256, 512, 1024, 2048, 4096)
 
 
 def make_chains(lengths, interp=O.CUBIC, dtype=np.uint16, dc=False):
+    """interp None: no resampling."""
     chains = {}
     for n in lengths:
         p = db_params(O.ChainParams(signal_length=n, ascans_per_bscan=64, bscans_per_buffer=1,
-                                    resampling=True, interpolation=interp, windowing=True, dispersion=True,
+                                    resampling=interp is not None, interpolation=interp or 0, windowing=True,
+                                    dispersion=True,
                                     dc_removal=dc, dc_window=32))
         t = standard_tables(n)
         # the default launch runs the FFT schedule of the ragged body of
@@ -57,13 +59,15 @@ def run_batch(chains, reqs, chunk=0):
     return batch
 
 
-@pytest.mark.parametrize("dtype,dc", [(np.uint16, False), (np.float32, True)])
-def test_ragged_matches_per_request_processing(dtype, dc):
+# (uint16, DC, no resampling): the banded DC path at N = 1024 and 2048
+@pytest.mark.parametrize("dtype,dc,interp", [(np.uint16, False, O.CUBIC), (np.float32, True, O.CUBIC),
+                                             (np.uint16, True, None)])
+def test_ragged_matches_per_request_processing(dtype, dc, interp):
     rng = np.random.default_rng(5)
-    chains = make_chains(LENGTHS, dtype=dtype, dc=dc)
+    chains = make_chains(LENGTHS, interp=interp, dtype=dtype, dc=dc)
     reqs = make_requests(rng, LENGTHS, 24, 90, dtype)
-    print(f"[ragged] {len(reqs)} requests, {sum(r['a'] for r in reqs)} A-scans, dtype {np.dtype(dtype)}, dc={dc}",
-          flush=True)
+    print(f"[ragged] {len(reqs)} requests, {sum(r['a'] for r in reqs)} A-scans, dtype {np.dtype(dtype)}, dc={dc}, "
+          f"interpolation={interp}", flush=True)
     batch = run_batch(chains, reqs, chunk=16)  # several tasks per request
     worst_exact, worst_oracle = 0.0, 0.0
     for i, r in enumerate(reqs):
