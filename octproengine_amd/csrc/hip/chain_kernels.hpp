@@ -1064,14 +1064,15 @@ __device__ __forceinline__ void chain_body(const ChainArgs A, float2* bufA, floa
     // the last time before the last pass's barrier).  Not where the staging
     // shares buffer B with the last reads.  (Pairs have a staging array of their own; their split exchange
     // after the FFT uses the FFT buffers.)
-    // The banded DC path (windows of 1 .. DC_BAND samples) needs more
-    // registers than the kernels with resampling taps have to spare at three
-    // wavefronts per SIMD: only the kernels without taps take it.  Staged
+    // The banded DC path (windows of 1 .. DC_BAND samples) needs ~29 more
+    // registers at the staging point: the kernels with up to 2 taps take it
+    // (linear spills 8 bytes per lane at N = 2048 and still gains 9 %),
+    // cubic and Lanczos-8 would spill 50-100 and run slower.  Staged
     // inside the FFT, it splits: the prefix sums go in there, the window
     // means after the output stores, where the FFT's registers are free (the
     // raw words of the staged group stay in registers in between).  The
     // general path stages whole inside the FFT.
-    constexpr bool BANDED = T == 1;
+    constexpr bool BANDED = T <= 2;
     constexpr bool SPLIT = DC && BANDED && dc_banded<N, TPA, DcSum>();
     constexpr bool PIPE = DC ? L::DC_PIPE : !L::SWAP_TOP_BARRIER;
     auto stage_prefix = [&]() {

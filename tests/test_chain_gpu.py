@@ -122,27 +122,28 @@ def test_dc_removal(dtype, window):
     assert e <= DB_TOL
 
 
-# The banded DC path: kernels without resampling taps, 8/16-bit samples,
-# N >= 1024 and windows of 1 .. 64 samples.  Every window residue mod 8
+# The banded DC path: kernels with up to 2 resampling taps (none, linear),
+# 8/16-bit samples, N >= 1024 and windows of 1 .. 64 samples.  Every window residue mod 8
 # (each a different placement of the padded prefix blocks), the widest band
 # windows, and 0 and 65 on the general path; the clipped windows at both
 # ends of every A-scan are part of each case.  Peak-normalised linear error
 # and dB error near the peak against the FP64 oracle.
+@pytest.mark.parametrize("method", ["none", "linear"])
 @pytest.mark.parametrize("dtype", [np.uint8, np.uint16])
 @pytest.mark.parametrize("window", [0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 31, 56, 63, 64, 65])
 @pytest.mark.parametrize("n", [1024, 2048])
-def test_dc_removal_banded_no_resampling(n, window, dtype):
+def test_dc_removal_banded(n, window, dtype, method):
     a = 23
     raw = _typed(S.fringes(n, a, seed=7 * n + window), dtype)
     t = standard_tables(n)
     for make, err, tol in ((lin_params, lin_error, LIN_TOL), (db_params, db_error, DB_TOL)):
-        p = make(_params(n, a, "none", True, dc_removal=True, dc_window=window))
+        p = make(_params(n, a, method, True, dc_removal=True, dc_window=window))
         g = GpuChain(p, t.curve, t.window, t.phasor, dtype=dtype)
         gpu = g.frame(raw)
         g.close()
         ref = O.process_frame(raw, p, t.curve, t.window, t.phasor)
         e = err(gpu, ref)
-        print(f"banded DC N={n} W={window} {np.dtype(dtype).name} {make.__name__}: {e:.2e}")
+        print(f"banded DC N={n} {method} W={window} {np.dtype(dtype).name} {make.__name__}: {e:.2e}")
         assert e <= tol
 
 
@@ -152,7 +153,7 @@ def test_dc_removal_banded_bitshift(window):
     n, a = 2048, 17
     raw = S.fringes(n, a, seed=3 + window)
     t = standard_tables(n)
-    p = lin_params(_params(n, a, "none", True, dc_removal=True, dc_window=window, bitshift=True))
+    p = lin_params(_params(n, a, "linear", True, dc_removal=True, dc_window=window, bitshift=True))
     g = GpuChain(p, t.curve, t.window, t.phasor)
     gpu = g.frame(raw)
     g.close()
