@@ -674,7 +674,8 @@ enum : int {
 // the clipped windows at both ends fall on one sample per lane (i = 0 and
 // i = 7) as long as W <= TPA - 2.  For i = 1..6 the windows lie inside the
 // A-scan (cnt = 2W): the exact integer numerator x*2W - sum is scaled by
-// the reciprocal of 2W (within one ulp); the end samples divide exactly.
+// the reciprocal of 2W (within one ulp); the end samples take the reciprocal
+// of cnt with one correction step (within one ulp).
 //
 // Q holds N entries of S, and dc_prefix_ints ints where the banded path can
 // run.  The prefix half contains two workgroup barriers (one for
@@ -884,14 +885,15 @@ __device__ __forceinline__ void dc_means_stage(const float (&xv)[8], int t, int 
         const S sum = Q[end] - (start > 0 ? Q[start - 1] : S(0));
         const int cnt = end - start + 1;
         const S x = (S)xs[n];
-        // x - sum/cnt with a single rounding of the result; windows inside
-        // the A-scan as in the band (reciprocal of 2W), so that wavefronts
-        // without a clipped window skip the division
+        // x - sum/cnt from the exact numerator: the reciprocal of cnt and one
+        // correction step (within one ulp), the same code for clipped and
+        // inside windows
         if constexpr (sizeof(S) == 4) {
-            if (inside)
-                xs[n] = (float)(__mul24(x, 2 * W) - sum) * (1.0f / (float)(2 * W));
-            else
-                xs[n] = (float)(__mul24(x, cnt) - sum) / (float)cnt;  // numerator exact
+            const float c = (float)cnt;
+            const float num = (float)(__mul24(x, cnt) - sum);
+            const float rc = __builtin_amdgcn_rcpf(c);
+            const float q0 = num * rc;
+            xs[n] = fmaf(fmaf(-q0, c, num), rc, q0);
         } else {
             xs[n] = (float)(x - sum / (double)cnt);
         }
