@@ -167,6 +167,36 @@ OPE_HIP_API int ope_hip_get_bg_profile(ope_hip_ctx* ctx, float* profile, size_t 
                                        size_t* length_out);
 OPE_HIP_API int ope_hip_request_bg_recording(ope_hip_ctx* ctx);
 
+/* ---- Sample offset.  No counterpart in the reference, whose chain
+ * converts the samples to float and carries their constant level (the
+ * fringe's DC, ~2e4 for 16-bit cameras) through every float operation
+ * (cpu_kernels.tpp:24-54, 84-267).  Its rounding then reaches every bin of
+ * the float FFT; once an FPN profile removes the DC bins
+ * (cpu_backend.cpp:246-281) weak bins within 60 dB of the peak carry it.
+ * With an offset m0 the chain runs on x - m0 (exact for 8/16-bit samples,
+ * m0 an integer) and the result is corrected by m0 * IFFT(S * phasor), S[j]
+ * the sum of output j's tap weights, evaluated in FP64 and folded into the
+ * epilogue's FPN constant: the outputs are those of the same chain on x,
+ * with less rounding.  The FPN statistics add the term back, so profiles are
+ * those of the chain on x.  Inactive while DC removal is enabled (that
+ * removes the level itself).
+ *   OPE_HIP_OFFSET_OFF:   m0 = 0 (the reference's arithmetic);
+ *   OPE_HIP_OFFSET_AUTO:  default; on the next frame m0 becomes the mean of
+ *                         its first (up to 32) A-scans, rounded to an
+ *                         integer for integer samples, and is kept;
+ *   OPE_HIP_OFFSET_FIXED: m0 = value (an integer for integer samples).
+ * Frames on different caller streams are not ordered with the frame that
+ * determines m0: submit that first frame before others that run
+ * concurrently on another stream.  A ragged batch built while an automatic
+ * offset is still undetermined runs without it.
+ * get: *mode and the offset in use (0 while undetermined); waits for
+ * in-flight frames. */
+#define OPE_HIP_OFFSET_OFF 0
+#define OPE_HIP_OFFSET_AUTO 1
+#define OPE_HIP_OFFSET_FIXED 2
+OPE_HIP_API int ope_hip_set_sample_offset(ope_hip_ctx* ctx, int mode, float value);
+OPE_HIP_API int ope_hip_get_sample_offset(ope_hip_ctx* ctx, int* mode, float* value);
+
 /* Device-resident processing: runs the whole chain over `n_ascans` A-scans
  * (a multiple of ascans_per_bscan is not required) from d_in (raw samples,
  * input_type, [n_ascans][N]) to d_out (float, [n_ascans][N/2]) on `stream`

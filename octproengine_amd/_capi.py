@@ -25,6 +25,8 @@ DTYPE_CODES = {np.dtype(np.uint8): UINT8, np.dtype(np.uint16): UINT16,
                np.dtype(np.int16): INT16, np.dtype(np.int32): INT32,
                np.dtype(np.float32): FLOAT32}
 LINEAR, CUBIC, LANCZOS = 0, 1, 2
+# sample offset modes (ope_hip_set_sample_offset)
+OFFSET_OFF, OFFSET_AUTO, OFFSET_FIXED = 0, 1, 2
 
 #: Every function include/ope_hip.h declares, with its ctypes signature.
 SIGNATURES = {
@@ -43,6 +45,8 @@ SIGNATURES = {
     "ope_hip_set_bg_profile": (c_int, [c_void_p, c_void_p, c_size_t]),
     "ope_hip_get_bg_profile": (c_int, [c_void_p, c_void_p, c_size_t, POINTER(c_size_t)]),
     "ope_hip_request_bg_recording": (c_int, [c_void_p]),
+    "ope_hip_set_sample_offset": (c_int, [c_void_p, c_int, c_float]),
+    "ope_hip_get_sample_offset": (c_int, [c_void_p, POINTER(c_int), POINTER(c_float)]),
     "ope_hip_process_device": (c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_void_p]),
     "ope_hip_submit_host": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                     c_void_p]),
@@ -256,6 +260,15 @@ class Context:
 
     def synchronize(self) -> None:
         self._check(lib.ope_hip_synchronize(self._h))
+
+    def set_sample_offset(self, mode: int, value: float = 0.0) -> None:
+        self._check(lib.ope_hip_set_sample_offset(self._h, int(mode), float(value)))
+
+    def get_sample_offset(self):
+        """(mode, offset in use; 0 while undetermined)."""
+        mode, value = c_int(0), c_float(0.0)
+        self._check(lib.ope_hip_get_sample_offset(self._h, ctypes.byref(mode), ctypes.byref(value)))
+        return int(mode.value), float(value.value)
 
     def autotune(self, d_in: int = 0, d_out: int = 0, n_ascans: int = 0, stream: int = 0):
         """Times the chain's launch variants on d_in -> d_out (the context's

@@ -82,7 +82,7 @@ __global__ void __launch_bounds__(Geo<N>::WG) autofocus_kernel(AutofocusArgs A) 
         uint32_t raw[8] = {0, 0, 0, 0, 0, 0, 0, 0};
         if (active) load_raw<IW>(A.in, N, a, t, raw);
         float xv[8];
-        convert_raw<IW>(ieee, A.bitshift != 0, raw, xv);
+        convert_raw<IW>(ieee, A.bitshift != 0, 0.f, raw, xv);
         __syncthreads();  // the previous A-scan's LDS reads are done
         if constexpr (DC) {
             dc_remove_stage<N, TPA, typename L::DcSum>(xv, t, A.dc_window,

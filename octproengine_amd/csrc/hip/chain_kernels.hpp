@@ -158,6 +158,8 @@ enum : int {
     F_AUTOMM = 32,   // log scaling with per-A-scan min/max
     F_BITSHIFT = 64, // input samples >> 4 (8/16-bit) or * 2^-32 (32-bit integers)
     F_FLIP = 128,    // reverse the A-scan order of B-scans 0, 2, 4, ...
+    F_OFFSET = 256,  // samples minus *offset; the epilogue constant `fpn` then
+                     // holds fpn - offset * IFFT(S * phasor) (see ChainArgs)
 };
 
 struct ChainArgs {
@@ -179,6 +181,13 @@ struct ChainArgs {
     float bg_w, bg_off;
     int ascans_per_bscan;    // B-scan flip: B-scan length
     long long flip_bscans;   // B-scans b with b even and b + 1 < flip_bscans are flipped
+    // Sample offset (F_OFFSET): the chain runs on x - m0, m0 = *offset, so
+    // that the float arithmetic does not carry the fringe's constant level;
+    // by linearity X = X' + m0 * D with D[k] = IFFT(S * phasor)[k], S[j] the
+    // sum of output j's tap weights.  The host folds -m0 * D (FP64) into the
+    // epilogue constant `fpn`; the statistics mode adds m0 * D back itself.
+    const float* offset;     // device scalar m0
+    const double2* offset_tab;  // [N/2] D (statistics mode)
     // FPN statistics mode
     double* acc;             // [8][N/2][3] (sum re, sum im, sum |X|^2)
     long long row0;          // accumulated row index of A-scan 0 of this call
@@ -245,9 +254,12 @@ __device__ __forceinline__ void load_raw(const void* in, int n, long long a, int
 // 4-byte words are IEEE floats when `ieee` is set (FLOAT32 input).  `shift`
 // is the CUDA backend's bitshift option (cuda_kernels.cu:45-70): 8/16-bit
 // samples >> 4, 32-bit integers * 2^-32 (exact: a power-of-two scaling of
-// the rounded value); IEEE floats are not shifted.
+// the rounded value); IEEE floats are not shifted.  Then the sample offset
+// m0 is subtracted (0 without F_OFFSET): exact for 8/16-bit samples, whose
+// m0 is an integer.
 template <int IW>
-__device__ __forceinline__ void convert_raw(bool ieee, bool shift, const uint32_t (&raw)[8], float (&x)[8]) {
+__device__ __forceinline__ void convert_raw(bool ieee, bool shift, float m0, const uint32_t (&raw)[8],
+                                            float (&x)[8]) {
     // 8/16-bit fields convert with one sub-dword-select conversion each; the
     // shift then is an exact float division by 16 and floor (v < 2^16).
     if constexpr (IW == 2) {
@@ -272,6 +284,22 @@ __device__ __forceinline__ void convert_raw(bool ieee, bool shift, const uint32_
 #pragma unroll
         for (int i = 0; i < 8; ++i) x[i] = ieee ? __uint_as_float(raw[i]) : (float)raw[i] * scale;
     }
+#pragma unroll
+    for (int i = 0; i < 8; ++i) x[i] -= m0;
+}
+
+// One sample as convert_raw converts it, before the offset.
+__device__ __forceinline__ float convert_sample(const void* in, long long i, int iw, bool ieee, bool shift) {
+    if (iw == 1) {
+        const unsigned x = static_cast<const uint8_t*>(in)[i];
+        return (float)(shift ? x >> 4 : x);
+    }
+    if (iw == 2) {
+        const unsigned x = static_cast<const uint16_t*>(in)[i];
+        return (float)(shift ? x >> 4 : x);
+    }
+    const uint32_t x = static_cast<const uint32_t*>(in)[i];
+    return ieee ? __uint_as_float(x) : (shift ? (float)x * 2.3283064365386963e-10f : (float)x);
 }
 
 // ------------------------------------------------------------------ scans
@@ -992,6 +1020,7 @@ __device__ __forceinline__ void chain_body(const ChainArgs A, float2* bufA, floa
     const int flags = A.flags;
     const bool ieee = A.in_type == IN_F32;
     const bool shift = (flags & F_BITSHIFT) != 0;
+    const float m0 = (flags & F_OFFSET) ? *gmem(A.offset) : 0.f;
 
     // ---- per-lane tables, read once per call
     // tap windows as LDS pointers into the lane's first staging buffer (the
@@ -1023,7 +1052,7 @@ __device__ __forceinline__ void chain_body(const ChainArgs A, float2* bufA, floa
 #pragma unroll
         for (int o = 0; o < 4; ++o) {
             const int k = t + o * TPA;
-            if (flags & F_FPN) {
+            if (flags & (F_FPN | F_OFFSET)) {
                 // pairs split into twice the spectrum (exact scaling)
                 float2 f = gload2(A.fpn, k);
                 fp[o] = PAIR ? cf{2.f * f.x, 2.f * f.y} : cf{f.x, f.y};
@@ -1035,6 +1064,19 @@ __device__ __forceinline__ void chain_body(const ChainArgs A, float2* bufA, floa
     }
 
     // LDS base: every pass reads positions t + e*TPA (base + constant)
+
+    // statistics of X = X' + m0 * D, the spectrum without the offset
+    double2 dsh[4];
+    if constexpr (STATS) {
+#pragma unroll
+        for (int o = 0; o < 4; ++o) {
+            dsh[o] = make_double2(0.0, 0.0);
+            if (flags & F_OFFSET) {
+                const OPE_GLOBAL double* d = gmem(reinterpret_cast<const double*>(A.offset_tab)) + 2 * (t + o * TPA);
+                dsh[o] = make_double2((double)m0 * d[0], (double)m0 * d[1]);
+            }
+        }
+    }
 
     double sx[AS][4], sy[AS][4], sxx[AS][4];
     int cur_seg[AS];
@@ -1081,7 +1123,7 @@ __device__ __forceinline__ void chain_body(const ChainArgs A, float2* bufA, floa
 #pragma unroll
         for (int u = 0; u < AS; ++u) {
             float x8[8];
-            convert_raw<IW>(ieee, shift, raw[u], x8);
+            convert_raw<IW>(ieee, shift, m0, raw[u], x8);
             dc_prefix_stage<N, TPA, DcSum, BANDED>(x8, t, A.dc_window, reinterpret_cast<DcSum*>(pa[u]),
                                                    scan_part + u * NW, xs[u]);
         }
@@ -1090,7 +1132,7 @@ __device__ __forceinline__ void chain_body(const ChainArgs A, float2* bufA, floa
 #pragma unroll
         for (int u = 0; u < AS; ++u) {
             float x8[8];
-            convert_raw<IW>(ieee, shift, raw[u], x8);
+            convert_raw<IW>(ieee, shift, m0, raw[u], x8);
             if constexpr (SPLIT) {
                 dc_means_stage<N, TPA, DcSum, BANDED>(x8, t, A.dc_window, reinterpret_cast<DcSum*>(pa[u]), xs[u]);
             } else if constexpr (DC) {
@@ -1119,7 +1161,7 @@ __device__ __forceinline__ void chain_body(const ChainArgs A, float2* bufA, floa
         if constexpr (!PIPE) {
 #pragma unroll
             for (int u = 0; u < AS; ++u) {
-                convert_raw<IW>(ieee, shift, raw[u], xv[u]);
+                convert_raw<IW>(ieee, shift, m0, raw[u], xv[u]);
                 if constexpr (!DC) {
                     float4* dst = reinterpret_cast<float4*>(xs[u] + 8 * t);
                     dst[0] = make_float4(xv[u][0], xv[u][1], xv[u][2], xv[u][3]);
@@ -1240,7 +1282,7 @@ __device__ __forceinline__ void chain_body(const ChainArgs A, float2* bufA, floa
                     }
 #pragma unroll
                     for (int o = 0; o < 4; ++o) {
-                        const double re = v[u][o].x, im = v[u][o].y;
+                        const double re = (double)v[u][o].x + dsh[o].x, im = (double)v[u][o].y + dsh[o].y;
                         sx[u][o] += re;
                         sy[u][o] += im;
                         sxx[u][o] += re * re + im * im;

@@ -12,10 +12,20 @@ namespace ope_hip {
 
 // ------------------------------------------------------------------ FPN select
 
+// Epilogue constant of the sample offset: fl(fpn[k] - m0 * D[k]) in FP64
+// (fpn taken as 0 when the profile is not applied).
+__device__ __forceinline__ float2 offset_epilogue(float2 fpn, bool use_fpn, float m0, double2 d) {
+    const double fx = use_fpn ? (double)fpn.x : 0.0, fy = use_fpn ? (double)fpn.y : 0.0;
+    return make_float2((float)(fx - (double)m0 * d.x), (float)(fy - (double)m0 * d.y));
+}
+
 // Minimum-variance segment mean per bin, the float/double mixture of
 // getMinimumVarianceMean (cpu_kernels.tpp:402-458).  acc is [segs][width][3].
+// With `ep` set, the offset's epilogue constant of the new profile is
+// written too (m0 = *offset, D = offset_tab).
 __global__ void fpn_select_kernel(const double* __restrict__ acc, int width, int segments,
-                                  int seg_w, float2* __restrict__ out) {
+                                  int seg_w, float2* __restrict__ out, const float* __restrict__ offset,
+                                  const double2* __restrict__ offset_tab, int use_fpn, float2* __restrict__ ep) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= width) return;
     const float factor = 1.0f / (float)seg_w;
@@ -32,6 +42,46 @@ __global__ void fpn_select_kernel(const double* __restrict__ acc, int width, int
         }
     }
     out[k] = make_float2(bx, by);
+    if (ep) ep[k] = offset_epilogue(make_float2(bx, by), use_fpn != 0, *offset, offset_tab[k]);
+}
+
+// Sample offset of a context, one workgroup of 256 lanes.  With
+// `determine` set, m0 becomes the mean of the first `rows` A-scans of `in`
+// (converted as the chain converts them), rounded to an integer for
+// integer samples (so that x - m0 is exact), and is stored to *offset;
+// otherwise *offset is kept.  Then the epilogue constant of every bin k <
+// width is rebuilt from the current profile: ep[k] = fl(fpn[k] - m0*D[k]).
+__global__ void __launch_bounds__(256) offset_prepare_kernel(const void* __restrict__ in, int n, int rows, int iw,
+                                                             int ieee, int shift, int determine,
+                                                             float* __restrict__ offset,
+                                                             const float2* __restrict__ fpn, int use_fpn,
+                                                             const double2* __restrict__ offset_tab,
+                                                             float2* __restrict__ ep, int width) {
+    __shared__ double part[256];
+    __shared__ float m0s;
+    if (determine) {
+        const long long total = (long long)rows * n;
+        double s = 0.0;
+        for (long long i = threadIdx.x; i < total; i += 256) s += (double)convert_sample(in, i, iw, ieee != 0, shift != 0);
+        part[threadIdx.x] = s;
+        __syncthreads();
+        for (int h = 128; h > 0; h >>= 1) {
+            if ((int)threadIdx.x < h) part[threadIdx.x] += part[threadIdx.x + h];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            const double mean = total > 0 ? part[0] / (double)total : 0.0;
+            float m0 = ieee ? (float)mean : (float)rint(mean);
+            if (!isfinite(m0)) m0 = 0.f;
+            *offset = m0;
+            m0s = m0;
+        }
+    } else if (threadIdx.x == 0) {
+        m0s = *offset;
+    }
+    __syncthreads();
+    const float m0 = m0s;
+    for (int k = threadIdx.x; k < width; k += 256) ep[k] = offset_epilogue(fpn[k], use_fpn != 0, m0, offset_tab[k]);
 }
 
 // Segment sums over arbitrary interleaved complex rows [height][width]
@@ -122,18 +172,7 @@ __global__ void op_convert_kernel(const void* __restrict__ in, float* __restrict
                                   int ieee, int shift) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= samples) return;
-    float v;
-    if (iw == 1) {
-        const unsigned x = static_cast<const uint8_t*>(in)[i];
-        v = (float)(shift ? x >> 4 : x);
-    } else if (iw == 2) {
-        const unsigned x = static_cast<const uint16_t*>(in)[i];
-        v = (float)(shift ? x >> 4 : x);
-    } else {
-        const uint32_t x = static_cast<const uint32_t*>(in)[i];
-        v = ieee ? __uint_as_float(x) : (shift ? (float)x * 2.3283064365386963e-10f : (float)x);
-    }
-    out[i] = v;
+    out[i] = convert_sample(in, i, iw, ieee != 0, shift != 0);
 }
 
 // y[j] = sum_i w[j][i] x[base[j] + i], j < count (banded tap table).

@@ -10,6 +10,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
+#include <complex>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -107,12 +109,29 @@ struct ope_hip_ctx {
     // autofocus: window used by the differentiable chain, FP64 sums
     float* d_af_window = nullptr;
     double* d_af_acc = nullptr;
+
+    // sample offset (ope_hip_set_sample_offset): mode, whether an automatic
+    // offset is still to be determined, the device scalar m0, the FP64 table
+    // D = IFFT(S * phasor) for k < N/2 and the epilogue constant
+    // fl(fpn - m0 * D); S[j] = sum of output j's tap weights
+    int offset_mode = OPE_HIP_OFFSET_AUTO;
+    bool offset_pending = true;
+    float* d_offset = nullptr;
+    double2* d_offset_tab = nullptr;
+    float2* d_ep = nullptr;
+    std::vector<double> tap_sums;
+
+    // set when work on caller streams (process_device, graphs, ragged
+    // batches) may still read the context's tables: the table setters then
+    // wait for the whole device, not only the context's own streams
+    std::atomic<bool> foreign_work{false};
 };
 
 // A ragged batch: snapshot of the class arguments, the task list and the
 // self-resetting counters of ragged_kernel.
 struct ope_hip_ragged {
     int device = 0;
+    std::vector<ope_hip_ctx*> ctxs;  // whose tables the batch reads
     std::string err;
     int n_reqs = 0;
     int n_tasks = 0;
@@ -132,6 +151,7 @@ struct ope_hip_ragged {
 // Frames of one context captured into a HIP graph.
 struct ope_hip_graph {
     int device = 0;
+    ope_hip_ctx* ctx = nullptr;  // whose tables the frames read
     std::string err;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -287,14 +307,16 @@ int build_taps(ope_hip_ctx* c, std::vector<int>& base, std::vector<float>& w) {
                           c->p.windowing_enabled ? c->curve_window.data() : nullptr, base, w);
 }
 
+int sync_all(ope_hip_ctx* c);
+
 int upload_taps(ope_hip_ctx* c) {
     c->taps = taps_for(c->p);
     std::vector<int> base;
     std::vector<float> w;
     int rc = build_taps(c, base, w);
     if (rc) return rc;
-    HIP_TRY(c, hipSetDevice(c->device));
-    HIP_TRY(c, hipStreamSynchronize(c->s_comp));
+    rc = sync_all(c);
+    if (rc) return rc;
     if (w.size() > c->tap_w_cap) {
         if (c->d_tap_w) HIP_TRY(c, hipFree(c->d_tap_w));
         c->d_tap_w = nullptr;
@@ -303,6 +325,97 @@ int upload_taps(ope_hip_ctx* c) {
     }
     HIP_TRY(c, hipMemcpy(c->d_tap_base, base.data(), base.size() * sizeof(int), hipMemcpyHostToDevice));
     HIP_TRY(c, hipMemcpy(c->d_tap_w, w.data(), w.size() * sizeof(float), hipMemcpyHostToDevice));
+    // the offset table's S[j]: the float weights as uploaded, summed in FP64
+    c->tap_sums.assign(c->n, 0.0);
+    for (int j = 0; j < c->n; ++j)
+        for (int i = 0; i < c->taps; ++i) c->tap_sums[j] += (double)w[(size_t)j * c->taps + i];
+    return OPE_HIP_OK;
+}
+
+// ---------------------------------------------------------------- sample offset
+
+// Unnormalised inverse DFT in FP64, X[k] = sum_j a[j] exp(+2 pi i j k / N),
+// in place (iterative radix 2; N a power of two).  Twiddles come from cos
+// and sin of each angle, so the error stays within a few ulp of |a|_1.
+void idft64(std::vector<std::complex<double>>& a) {
+    const int n = (int)a.size();
+    for (int i = 1, j = 0; i < n; ++i) {
+        int bit = n >> 1;
+        for (; j & bit; bit >>= 1) j ^= bit;
+        j ^= bit;
+        if (i < j) std::swap(a[i], a[j]);
+    }
+    std::vector<std::complex<double>> tw(n / 2);
+    for (int k = 0; k < n / 2; ++k) {
+        const double ang = 2.0 * M_PI * (double)k / (double)n;
+        tw[k] = std::complex<double>(std::cos(ang), std::sin(ang));
+    }
+    for (int len = 2; len <= n; len <<= 1) {
+        const int h = len / 2, stride = n / len;
+        for (int i = 0; i < n; i += len)
+            for (int k = 0; k < h; ++k) {
+                const std::complex<double> u = a[i + k], v = a[i + k + h] * tw[(size_t)k * stride];
+                a[i + k] = u + v;
+                a[i + k + h] = u - v;
+            }
+    }
+}
+
+bool offset_active(const ope_hip_ctx* c) {
+    return c->offset_mode != OPE_HIP_OFFSET_OFF && !c->p.dc_removal_enabled;
+}
+
+bool fpn_applied(const ope_hip_ctx* c) { return c->p.fpn_removal && c->have_fpn; }
+
+// Frames with the offset: samples minus m0, the epilogue constant in place
+// of the FPN profile, D for the statistics mode.
+void apply_offset(ope_hip_ctx* c, ChainArgs& a) {
+    a.flags |= F_OFFSET;
+    a.offset = c->d_offset;
+    a.offset_tab = c->d_offset_tab;
+    a.fpn = c->d_ep;
+}
+
+// Rebuilds the offset's tables after a change of the taps, the phasor, the
+// dispersion flag, the FPN profile or m0: D (when `table`) on the host in
+// FP64, then the epilogue constant fl(fpn - m0 * D) on the device.  Waits
+// for it, so frames on any stream see the new constant.
+int refresh_offset(ope_hip_ctx* c, bool table) {
+    if (c->offset_mode == OPE_HIP_OFFSET_OFF) return OPE_HIP_OK;
+    const int n = c->n, half = n / 2;
+    HIP_TRY(c, hipSetDevice(c->device));
+    if (table) {
+        const bool disp = c->p.dispersion_enabled && c->phasor_valid;
+        std::vector<std::complex<double>> a(n);
+        for (int j = 0; j < n; ++j) {
+            const std::complex<double> ph =
+                disp ? std::complex<double>(c->curve_phasor[2 * (size_t)j], c->curve_phasor[2 * (size_t)j + 1])
+                     : std::complex<double>(1.0, 0.0);
+            a[j] = c->tap_sums[j] * ph;
+        }
+        idft64(a);
+        std::vector<double2> d(half);
+        for (int k = 0; k < half; ++k) d[k] = make_double2(a[k].real(), a[k].imag());
+        HIP_TRY(c, hipMemcpy(c->d_offset_tab, d.data(), half * sizeof(double2), hipMemcpyHostToDevice));
+    }
+    hipLaunchKernelGGL(offset_prepare_kernel, dim3(1), dim3(256), 0, c->s_comp, nullptr, n, 0, 2, 0, 0, 0,
+                       c->d_offset, c->d_fpn, fpn_applied(c) ? 1 : 0, c->d_offset_tab, c->d_ep, half);
+    HIP_TRY(c, hipGetLastError());
+    HIP_TRY(c, hipStreamSynchronize(c->s_comp));
+    return OPE_HIP_OK;
+}
+
+// Determines an automatic offset from the first (up to 32) A-scans of d_in
+// on stream s, and rebuilds the epilogue constant with it.
+int determine_offset(ope_hip_ctx* c, const void* d_in, long long n_ascans, hipStream_t s) {
+    const int n = c->n;
+    const int rows = (int)std::min<long long>(n_ascans, 32);
+    const int cls = input_class(c->p.input_type);
+    hipLaunchKernelGGL(offset_prepare_kernel, dim3(1), dim3(256), 0, s, d_in, n, rows,
+                       (int)input_bytes(c->p.input_type), cls == IN_F32 ? 1 : 0, c->p.bitshift ? 1 : 0, 1,
+                       c->d_offset, c->d_fpn, fpn_applied(c) ? 1 : 0, c->d_offset_tab, c->d_ep, n / 2);
+    HIP_TRY(c, hipGetLastError());
+    c->offset_pending = false;
     return OPE_HIP_OK;
 }
 
@@ -504,6 +617,13 @@ int run_frame(ope_hip_ctx* c, const void* d_in, float* d_out, long long n_ascans
     a.in = d_in;
     a.out = d_out;
     a.n_ascans = n_ascans;
+    if (offset_active(c)) {
+        if (c->offset_pending) {
+            int rc = determine_offset(c, d_in, n_ascans, s);
+            if (rc) return rc;
+        }
+        apply_offset(c, a);
+    }
 
     // FPN accumulation: only the first `required` accumulated A-scans ever
     // enter a determination (cpu_backend.cpp:324-330), and of those only
@@ -513,7 +633,7 @@ int run_frame(ope_hip_ctx* c, const void* d_in, float* d_out, long long n_ascans
     const long long rows_used = (long long)seg_w * kFpnSegments;
     if (seg_w > 0 && c->fpn_count < rows_used) {
         ChainArgs st = a;
-        st.flags = a.flags & F_DISP;
+        st.flags = a.flags & (F_DISP | F_OFFSET);
         st.row0 = c->fpn_count;
         st.seg_w = seg_w;
         st.stat_rows = (int)std::min<long long>(n_ascans, rows_used - c->fpn_count);
@@ -523,8 +643,11 @@ int run_frame(ope_hip_ctx* c, const void* d_in, float* d_out, long long n_ascans
     c->fpn_count += n_ascans;
     if ((c->fpn_requested || p.fpn_continuous) && c->fpn_count >= required) {
         if (seg_w > 0) {
+            // with an offset mode the epilogue constant follows the profile
+            const bool ep = c->offset_mode != OPE_HIP_OFFSET_OFF;
             hipLaunchKernelGGL(fpn_select_kernel, dim3((half + 255) / 256), dim3(256), 0, s, c->d_acc, half,
-                               kFpnSegments, seg_w, c->d_fpn);
+                               kFpnSegments, seg_w, c->d_fpn, c->d_offset, c->d_offset_tab, p.fpn_removal ? 1 : 0,
+                               ep ? c->d_ep : nullptr);
             HIP_TRY(c, hipGetLastError());
         } else {
             // fewer than 8 rows per segment: the reference divides by zero;
@@ -532,6 +655,11 @@ int run_frame(ope_hip_ctx* c, const void* d_in, float* d_out, long long n_ascans
             std::vector<float2> nanv(half, make_float2(NAN, NAN));
             HIP_TRY(c, hipMemcpyAsync(c->d_fpn, nanv.data(), half * sizeof(float2), hipMemcpyHostToDevice, s));
             HIP_TRY(c, hipStreamSynchronize(s));
+            if (c->offset_mode != OPE_HIP_OFFSET_OFF) {
+                hipLaunchKernelGGL(offset_prepare_kernel, dim3(1), dim3(256), 0, s, nullptr, c->n, 0, 2, 0, 0, 0,
+                                   c->d_offset, c->d_fpn, p.fpn_removal ? 1 : 0, c->d_offset_tab, c->d_ep, half);
+                HIP_TRY(c, hipGetLastError());
+            }
         }
         c->have_fpn = true;
         c->fpn_pairs_stored = half;
@@ -597,7 +725,8 @@ void free_ctx(ope_hip_ctx* c) {
     }
     for (void* p : {(void*)c->d_tap_base, (void*)c->d_tap_w, (void*)c->d_phasor, (void*)c->d_twiddle,
                     (void*)c->d_fpn, (void*)c->d_bg, (void*)c->d_acc, (void*)c->d_af_window,
-                    (void*)c->d_af_acc, (void*)c->d_sin_curve, (void*)c->d_sin_tmp})
+                    (void*)c->d_af_acc, (void*)c->d_sin_curve, (void*)c->d_sin_tmp, (void*)c->d_offset,
+                    (void*)c->d_offset_tab, (void*)c->d_ep})
         if (p) (void)hipFree(p);
     for (hipStream_t s : {c->s_in, c->s_comp, c->s_out})
         if (s) (void)hipStreamDestroy(s);
@@ -616,6 +745,8 @@ void host_done_cb(void* u) {
 
 int sync_all(ope_hip_ctx* c) {
     HIP_TRY(c, hipSetDevice(c->device));
+    // frames on caller streams read the tables too
+    if (c->foreign_work.exchange(false)) HIP_TRY(c, hipDeviceSynchronize());
     HIP_TRY(c, hipStreamSynchronize(c->s_in));
     HIP_TRY(c, hipStreamSynchronize(c->s_comp));
     HIP_TRY(c, hipStreamSynchronize(c->s_out));
@@ -707,6 +838,7 @@ ChainArgs steady_args(ope_hip_ctx* c) {
     ChainArgs a = base_args(c);
     if (c->p.fpn_removal && c->have_fpn) a.flags |= F_FPN;
     if (c->p.bg_removal && c->have_bg) a.flags |= F_BG;
+    if (offset_active(c) && !c->offset_pending) apply_offset(c, a);
     return a;
 }
 
@@ -737,6 +869,7 @@ int build_ragged(ope_hip_ragged* r, ope_hip_ctx* const* ctxs, int n_ctx, const o
     }
     const ope_hip_ctx* c0 = ctxs[0];
     r->device = c0->device;
+    r->ctxs.assign(ctxs, ctxs + n_ctx);
     r->n_reqs = n_reqs;
     const size_t iw = input_bytes(c0->p.input_type);
     std::vector<RaggedRequest> hreq(n_reqs);
@@ -943,6 +1076,12 @@ int ope_hip_create(int device, const ope_hip_params* params, int n_slots, ope_hi
     CREATE_TRY(hipMalloc(&c->d_fpn, half * sizeof(float2)));
     CREATE_TRY(hipMalloc(&c->d_bg, half * sizeof(float)));
     CREATE_TRY(hipMalloc(&c->d_acc, (size_t)kFpnSegments * half * 3 * sizeof(double)));
+    CREATE_TRY(hipMalloc(&c->d_offset, sizeof(float)));
+    CREATE_TRY(hipMalloc(&c->d_offset_tab, half * sizeof(double2)));
+    CREATE_TRY(hipMalloc(&c->d_ep, half * sizeof(float2)));
+    CREATE_TRY(hipMemset(c->d_offset, 0, sizeof(float)));
+    CREATE_TRY(hipMemset(c->d_offset_tab, 0, half * sizeof(double2)));
+    CREATE_TRY(hipMemset(c->d_ep, 0, half * sizeof(float2)));
     {
         std::vector<float2> tw(n);
         for (int k = 0; k < n; ++k) {
@@ -986,6 +1125,7 @@ int ope_hip_create(int device, const ope_hip_params* params, int n_slots, ope_hi
         }
     }
     rc = upload_taps(c);
+    if (!rc) rc = refresh_offset(c, true);
     if (rc) return bail(rc);
     *out_ctx = c;
     return OPE_HIP_OK;
@@ -1004,6 +1144,8 @@ int ope_hip_set_params(ope_hip_ctx* c, const ope_hip_params* params) {
         params->bscans_per_buffer != c->p.bscans_per_buffer || params->input_type != c->p.input_type)
         return fail(c, OPE_HIP_EINVAL, "set_params cannot change dimensions or input type");
     std::lock_guard<std::mutex> lk(c->submit_mu);
+    rc = sync_all(c);
+    if (rc) return rc;
     ope_hip_params old = c->p;
     c->p = *params;
     if (old.resampling_enabled != params->resampling_enabled || old.interpolation != params->interpolation ||
@@ -1011,7 +1153,10 @@ int ope_hip_set_params(ope_hip_ctx* c, const ope_hip_params* params) {
         rc = upload_taps(c);
         if (rc) return rc;
     }
-    return OPE_HIP_OK;
+    // an automatic offset belongs to the samples as converted
+    if (old.bitshift != params->bitshift && c->offset_mode == OPE_HIP_OFFSET_AUTO) c->offset_pending = true;
+    // the dispersion flag and the FPN switch enter the offset's tables
+    return refresh_offset(c, true);
 }
 
 int ope_hip_set_resampling_curve(ope_hip_ctx* c, const float* curve, size_t length) {
@@ -1022,8 +1167,9 @@ int ope_hip_set_resampling_curve(ope_hip_ctx* c, const float* curve, size_t leng
     std::lock_guard<std::mutex> lk(c->submit_mu);
     c->curve_resample.assign(curve, curve + length);
     int rc = sync_all(c);
-    if (rc) return rc;
-    return upload_taps(c);
+    if (!rc) rc = upload_taps(c);
+    if (!rc) rc = refresh_offset(c, true);
+    return rc;
 }
 
 int ope_hip_set_window_curve(ope_hip_ctx* c, const float* curve, size_t length) {
@@ -1034,8 +1180,9 @@ int ope_hip_set_window_curve(ope_hip_ctx* c, const float* curve, size_t length) 
     std::lock_guard<std::mutex> lk(c->submit_mu);
     c->curve_window.assign(curve, curve + length);
     int rc = sync_all(c);
-    if (rc) return rc;
-    return upload_taps(c);
+    if (!rc) rc = upload_taps(c);
+    if (!rc) rc = refresh_offset(c, true);
+    return rc;
 }
 
 int ope_hip_set_dispersion_curve(ope_hip_ctx* c, const float* cos_sin, size_t length) {
@@ -1048,7 +1195,7 @@ int ope_hip_set_dispersion_curve(ope_hip_ctx* c, const float* cos_sin, size_t le
     c->phasor_valid = (length == 2 * (size_t)c->n);
     if (c->phasor_valid)
         HIP_TRY(c, hipMemcpy(c->d_phasor, cos_sin, length * sizeof(float), hipMemcpyHostToDevice));
-    return OPE_HIP_OK;
+    return refresh_offset(c, true);
 }
 
 int ope_hip_set_fpn_profile(ope_hip_ctx* c, const float* interleaved, size_t pairs) {
@@ -1064,7 +1211,7 @@ int ope_hip_set_fpn_profile(ope_hip_ctx* c, const float* interleaved, size_t pai
     HIP_TRY(c, hipMemcpy(c->d_fpn, prof.data(), half * sizeof(float2), hipMemcpyHostToDevice));
     c->have_fpn = pairs > 0;
     c->fpn_pairs_stored = pairs;
-    return OPE_HIP_OK;
+    return refresh_offset(c, false);
 }
 
 int ope_hip_get_fpn_profile(ope_hip_ctx* c, float* interleaved, size_t pairs_capacity, size_t* pairs_out) {
@@ -1130,6 +1277,38 @@ int ope_hip_request_bg_recording(ope_hip_ctx* c) {
     return OPE_HIP_OK;
 }
 
+int ope_hip_set_sample_offset(ope_hip_ctx* c, int mode, float value) {
+    if (!c) return OPE_HIP_EINVAL;
+    if (mode != OPE_HIP_OFFSET_OFF && mode != OPE_HIP_OFFSET_AUTO && mode != OPE_HIP_OFFSET_FIXED)
+        return fail(c, OPE_HIP_EINVAL, "unknown sample offset mode %d", mode);
+    if (mode == OPE_HIP_OFFSET_FIXED) {
+        if (!std::isfinite(value)) return fail(c, OPE_HIP_EINVAL, "sample offset must be finite");
+        if (input_class(c->p.input_type) != IN_F32 && value != std::rint(value))
+            return fail(c, OPE_HIP_EINVAL, "the sample offset of integer samples must be an integer");
+    }
+    std::lock_guard<std::mutex> lk(c->submit_mu);
+    int rc = sync_all(c);
+    if (rc) return rc;
+    c->offset_mode = mode;
+    c->offset_pending = mode == OPE_HIP_OFFSET_AUTO;
+    const float m0 = mode == OPE_HIP_OFFSET_FIXED ? value : 0.f;
+    HIP_TRY(c, hipMemcpy(c->d_offset, &m0, sizeof(float), hipMemcpyHostToDevice));
+    return refresh_offset(c, true);
+}
+
+int ope_hip_get_sample_offset(ope_hip_ctx* c, int* mode, float* value) {
+    if (!c) return OPE_HIP_EINVAL;
+    std::lock_guard<std::mutex> lk(c->submit_mu);
+    int rc = sync_all(c);
+    if (rc) return rc;
+    float m0 = 0.f;
+    if (c->offset_mode != OPE_HIP_OFFSET_OFF && !c->offset_pending)
+        HIP_TRY(c, hipMemcpy(&m0, c->d_offset, sizeof(float), hipMemcpyDeviceToHost));
+    if (mode) *mode = c->offset_mode;
+    if (value) *value = m0;
+    return OPE_HIP_OK;
+}
+
 int ope_hip_process_device(ope_hip_ctx* c, const void* d_in, float* d_out, int64_t n_ascans, void* stream) {
     if (!c) return OPE_HIP_EINVAL;
     if (n_ascans < 0) return fail(c, OPE_HIP_EINVAL, "n_ascans must be >= 0");
@@ -1139,6 +1318,7 @@ int ope_hip_process_device(ope_hip_ctx* c, const void* d_in, float* d_out, int64
     std::lock_guard<std::mutex> lk(c->submit_mu);
     HIP_TRY(c, hipSetDevice(c->device));
     hipStream_t s = static_cast<hipStream_t>(stream);  // NULL: the HIP null stream
+    c->foreign_work = true;
     return run_frame(c, d_in, d_out, n_ascans, s);
 }
 
@@ -1210,7 +1390,7 @@ int ope_hip_op_min_variance_mean(ope_hip_ctx* c, const float* input, int width, 
     }
     if (e == hipSuccess) {
         hipLaunchKernelGGL(fpn_select_kernel, dim3((width + 255) / 256), dim3(256), 0, c->s_comp, d_acc, width,
-                           segments, seg_w, d_out);
+                           segments, seg_w, d_out, nullptr, nullptr, 0, nullptr);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipStreamSynchronize(c->s_comp);
@@ -1445,9 +1625,22 @@ int ope_hip_graph_create(ope_hip_ctx* c, const void* const* d_in, float* const* 
         delete g;
         return code;
     };
+    g->ctx = c;
     hipError_t e = hipSetDevice(c->device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
     if (e != hipSuccess) return cleanup(OPE_HIP_ERUNTIME, "stream", e);
+    // an automatic offset is determined before the capture, from the first
+    // frame, so that every replay and later eager frame use the same one
+    if (offset_active(c) && c->offset_pending) {
+        if (determine_offset(c, d_in[0], ascans_per_frame, c->s_comp) != OPE_HIP_OK) {
+            const std::string msg = c->err;
+            cleanup(OPE_HIP_ERUNTIME, "offset", hipSuccess);
+            g_create_err = msg;
+            return OPE_HIP_ERUNTIME;
+        }
+        e = hipStreamSynchronize(c->s_comp);
+        if (e != hipSuccess) return cleanup(OPE_HIP_ERUNTIME, "offset", e);
+    }
     e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) return cleanup(OPE_HIP_ERUNTIME, "hipStreamBeginCapture", e);
     // the frame bookkeeping of run_frame is host state: keep it as it was
@@ -1472,6 +1665,7 @@ int ope_hip_graph_create(ope_hip_ctx* c, const void* const* d_in, float* const* 
 
 int ope_hip_graph_launch(ope_hip_graph* g, void* stream) {
     if (!g) return OPE_HIP_EINVAL;
+    g->ctx->foreign_work = true;
     hipError_t e = hipSetDevice(g->device);
     if (e == hipSuccess) e = hipGraphLaunch(g->exec, static_cast<hipStream_t>(stream));
     if (e != hipSuccess) {
@@ -1554,6 +1748,8 @@ int ope_hip_autotune(ope_hip_ctx* c, const void* d_in, float* d_out, int64_t n_a
         a.flip_bscans = n_ascans / p.ascans_per_bscan;
     }
     if (p.bg_removal && c->have_bg && !p.sinusoidal_correction) a.flags |= F_BG;
+    if (offset_active(c) && !c->offset_pending) apply_offset(c, a);
+    c->foreign_work = true;
     const int mode = p.dc_removal_enabled ? M_DC : 0;
     const int iw = (int)input_bytes(p.input_type);
     struct Cand {
@@ -1644,6 +1840,7 @@ int ope_hip_ragged_run(ope_hip_ragged* r, void* stream) {
     if (r->n_tasks == 0) return OPE_HIP_OK;
     hipError_t e = hipSetDevice(r->device);
     if (e != hipSuccess) return ragged_fail(r, OPE_HIP_ERUNTIME, "hipSetDevice: %s", hipGetErrorString(e));
+    for (ope_hip_ctx* c : r->ctxs) c->foreign_work = true;
     RaggedArgs a{r->d_cls, r->d_reqs, r->d_tasks, r->n_tasks, r->d_counters, r->d_remaining, r->d_finish,
                  r->d_start};
     hipLaunchKernelGGL(r->fn, dim3((unsigned)r->grid), dim3(kRaggedWG), 0, static_cast<hipStream_t>(stream), a);

@@ -29,7 +29,12 @@ def make_chains(lengths, interp=O.CUBIC, dtype=np.uint16, dc=False):
         t = standard_tables(n)
         # the default launch runs the FFT schedule of the ragged body of
         # each length (the swap schedule at N = 1024 and 2048)
-        chains[n] = (GpuChain(p, t.curve, t.window, t.phasor, dtype=dtype), p, t)
+        g = GpuChain(p, t.curve, t.window, t.phasor, dtype=dtype)
+        # a fixed sample offset, so that the batch (built before any frame)
+        # and the per-request frames use the same one
+        from octproengine_amd import _capi
+        g.ctx.set_sample_offset(_capi.OFFSET_FIXED, 20000.0 if dtype != np.float32 else 0.3)
+        chains[n] = (g, p, t)
     return chains
 
 
