@@ -237,6 +237,19 @@ OPE_HIP_API int ope_hip_host_free(void* ptr);
 OPE_HIP_API int ope_hip_op_min_variance_mean(ope_hip_ctx* ctx, const float* input, int width,
                                              int height, int segments, float* out);
 
+/* The same operation on device memory, allocating nothing.  No counterpart
+ * in the reference: it is for callers that cannot allocate (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * d_input: device, interleaved complex rows [height][width][2] (8-byte
+ * aligned); d_out: device, 2*width floats.  Asynchronous on `stream` (NULL:
+ * the HIP null stream).  Its segment sums live in a static device array of
+ * 2 MiB per GPU, so segments * width must not exceed 87381 (else
+ * OPE_HIP_EINVAL); the array is shared by every call on that GPU, so calls
+ * that overlap on one GPU race.  The context supplies the device (its
+ * creation takes the array's address there). */
+OPE_HIP_API int ope_hip_op_min_variance_mean_device(ope_hip_ctx* ctx, const float* d_input, int width,
+                                                    int height, int segments, float* d_out, void* stream);
+
 /* ---- Launch autotuning.  No counterpart in the reference (its CUDA
  * backend launches fixed grids, cuda_backend.cu:486-743).  Times the chain's
  * launch variants for the context's current configuration (interpolation,

@@ -52,6 +52,8 @@ SIGNATURES = {
                                     c_void_p]),
     "ope_hip_synchronize": (c_int, [c_void_p]),
     "ope_hip_op_min_variance_mean": (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_void_p]),
+    "ope_hip_op_min_variance_mean_device": (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_void_p,
+                                                    c_void_p]),
     "ope_hip_op_convert_input": (c_int, [c_void_p, c_void_p, c_int, c_int64, c_int, c_void_p]),
     "ope_hip_op_klinearization": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_void_p]),
     "ope_hip_op_windowing": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p]),
@@ -247,6 +249,13 @@ class Context:
         self._check(lib.ope_hip_op_min_variance_mean(self._h, inter.ctypes.data, w, h, segments,
                                                      out.ctypes.data))
         return out
+
+    def min_variance_mean_device(self, d_input: int, width: int, height: int, segments: int, d_out: int,
+                                 stream: int = 0) -> None:
+        """The allocation-free form on device memory: d_input interleaved
+        complex [height][width][2], d_out 2*width floats; asynchronous."""
+        self._check(lib.ope_hip_op_min_variance_mean_device(self._h, c_void_p(d_input), width, height, segments,
+                                                            c_void_p(d_out), c_void_p(stream or None)))
 
     def submit_host(self, host_in: int, host_out: int, on_done=None, on_input_free=None) -> None:
         """Host frame: H2D, chain, D2H on the context's streams; returns once

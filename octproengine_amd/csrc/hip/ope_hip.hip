@@ -28,7 +28,38 @@
 
 using namespace ope_hip;
 
+// Device scratch of ope_hip_op_min_variance_mean_device: the FP64 segment
+// sums [segments][width][3] of one call, so that the entry point allocates
+// nothing.  Each GPU holds its own copy for the life of the process; calls
+// on one GPU share it.  In host code `scratch` names a host-side stand-in:
+// its device address on each GPU comes from hipGetSymbolAddress, taken once
+// per device when a context is created there (scratch_init).
+constexpr size_t kScratchBytes = 2u << 20;
+constexpr size_t kScratchDoubles = kScratchBytes / sizeof(double);
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
 namespace {
+
+std::mutex g_scratch_mu;
+std::vector<double*> g_scratch_ptr;  // device address of `scratch`, per device id
+
+// Takes the address of `scratch` on `device` (which must be current) once.
+hipError_t scratch_init(int device) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (device < 0) return hipErrorInvalidDevice;
+    if ((size_t)device >= g_scratch_ptr.size()) g_scratch_ptr.resize((size_t)device + 1, nullptr);
+    if (g_scratch_ptr[device]) return hipSuccess;
+    double* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(reinterpret_cast<void**>(&ptr), HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[device] = ptr;
+    return hipSuccess;
+}
+
+double* scratch_ptr(int device) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return (device >= 0 && (size_t)device < g_scratch_ptr.size()) ? g_scratch_ptr[device] : nullptr;
+}
 
 constexpr int kFpnSegments = 8;  // cpu_backend.cpp:306
 
@@ -1061,6 +1092,7 @@ int ope_hip_create(int device, const ope_hip_params* params, int n_slots, ope_hi
         }                                                                                       \
     } while (0)
     CREATE_TRY(hipSetDevice(device));
+    CREATE_TRY(scratch_init(device));
     hipDeviceProp_t prop;
     CREATE_TRY(hipGetDeviceProperties(&prop, device));
     c->cu_count = prop.multiProcessorCount;
@@ -1367,6 +1399,7 @@ int ope_hip_op_min_variance_mean(ope_hip_ctx* c, const float* input, int width, 
         return fail(c, OPE_HIP_EINVAL, "width, height and segments must be > 0");
     const int seg_w = height / segments;
     if (seg_w <= 0) return fail(c, OPE_HIP_EINVAL, "height must be >= segments");
+    if (segments > 65535) return fail(c, OPE_HIP_EINVAL, "at most 65535 segments");
     std::lock_guard<std::mutex> lk(c->submit_mu);
     HIP_TRY(c, hipSetDevice(c->device));
     const size_t in_bytes = (size_t)seg_w * segments * width * 2 * sizeof(float);
@@ -1400,6 +1433,33 @@ int ope_hip_op_min_variance_mean(ope_hip_ctx* c, const float* input, int width, 
     return rc;
 }
 
+int ope_hip_op_min_variance_mean_device(ope_hip_ctx* c, const float* d_input, int width, int height, int segments,
+                                        float* d_out, void* stream) {
+    if (!c) return OPE_HIP_EINVAL;
+    if (!d_input || !d_out) return fail(c, OPE_HIP_EINVAL, "null pointer");
+    if (width <= 0 || height <= 0 || segments <= 0)
+        return fail(c, OPE_HIP_EINVAL, "width, height and segments must be > 0");
+    const int seg_w = height / segments;
+    if (seg_w <= 0) return fail(c, OPE_HIP_EINVAL, "height must be >= segments");
+    if (segments > 65535) return fail(c, OPE_HIP_EINVAL, "at most 65535 segments");
+    if ((size_t)segments * (size_t)width * 3 > kScratchDoubles)
+        return fail(c, OPE_HIP_EINVAL, "segments * width = %lld exceeds the scratch limit of %zu",
+                    (long long)segments * width, kScratchDoubles / 3);
+    if (reinterpret_cast<uintptr_t>(d_input) % 8 || reinterpret_cast<uintptr_t>(d_out) % 8)
+        return fail(c, OPE_HIP_EINVAL, "d_input and d_out must be 8-byte aligned");
+    double* acc = scratch_ptr(c->device);
+    if (!acc) return fail(c, OPE_HIP_ESTATE, "no scratch address for device %d", c->device);
+    HIP_TRY(c, hipSetDevice(c->device));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    // every sum the selection reads is written here first: nothing to clear
+    hipLaunchKernelGGL(segment_sums_kernel, dim3((width + 255) / 256, segments), dim3(256), 0, s,
+                       reinterpret_cast<const float2*>(d_input), width, seg_w, segments, acc);
+    HIP_TRY(c, hipGetLastError());
+    hipLaunchKernelGGL(fpn_select_kernel, dim3((width + 255) / 256), dim3(256), 0, s, acc, width, segments, seg_w,
+                       reinterpret_cast<float2*>(d_out), nullptr, nullptr, 0, nullptr);
+    HIP_TRY(c, hipGetLastError());
+    return OPE_HIP_OK;
+}
 
 int ope_hip_op_convert_input(ope_hip_ctx* c, const void* input, int input_type, int64_t samples, int bitshift,
                              float* out) {
