@@ -298,3 +298,242 @@ def test_sinusoidal_correction_reproduces_linear_fields():
     want = 2.5 * (curve[k.astype(int)].astype(np.float64) + l * a) + 1.0
     assert np.allclose(out[:-1, 0], want, rtol=0, atol=1e-9)
     assert np.array_equal(out[-1], field[-1])  # the last A-scan is not rewritten
+
+
+# ------------------------------------------------------------------ FPN accumulation across frames
+
+class ReferenceFpnReplay:
+    """Step-by-step replay of CpuBackend::Impl::computeFixedPatternNoiseIfRequested
+    (cpu_backend.cpp:305-357): every processed A-scan is pushed onto the
+    accumulation; a determination needs a request (or continuous mode) and at
+    least bscanCount * ascansPerBscan accumulated A-scans, and uses the first
+    that many; continuous mode then erases the newer entries, one-shot mode
+    clears everything.  Written from the reference, not from the oracle."""
+
+    def __init__(self):
+        self.accumulated = []
+        self.count = 0
+        self.requested = False
+        self.profile = None
+
+    def frame(self, X, bscan_count, ascans_per_bscan, continuous, n):
+        for row in X:
+            self.accumulated.append(row)
+            self.count += 1
+        required = bscan_count * ascans_per_bscan
+        if not (self.requested or continuous):
+            return
+        if self.count < required:
+            return
+        inputs = self.accumulated if self.count == required else self.accumulated[:required]
+        mean = O.min_variance_mean(np.asarray(inputs), 8)
+        self.profile = np.array([mean[i] for i in range(2 * (n // 2))], dtype=np.float32)
+        if continuous:
+            if self.count > required:
+                del self.accumulated[required:]
+                self.count = required
+        else:
+            self.accumulated = []
+            self.count = 0
+        self.requested = False
+
+
+def check_fpn_update(update):
+    n, a = 16, 16
+    rng = np.random.default_rng(77)
+    frames = [rng.normal(0, 1, (2 * a, n)) + 1j * rng.normal(0, 1, (2 * a, n)) + k for k in range(4)]
+    # (bscan count, continuous, frames before which a request is made)
+    for count, continuous, requests in ((2, False, {0, 3}), (3, True, set()), (2, False, {2}), (1, False, {1, 2})):
+        p = O.ChainParams(signal_length=n, ascans_per_bscan=a, bscans_per_buffer=2, fpn_bscan_count=count,
+                          fpn_continuous=continuous)
+        state, ref = O.ChainState(), ReferenceFpnReplay()
+        first = None
+        for f, X in enumerate(frames):
+            if f in requests:
+                state.fpn_requested = True
+                ref.requested = True
+            update(state, X, p)
+            ref.frame(X, count, a, continuous, n)
+            if ref.profile is None:
+                assert state.fpn_profile is None, (count, continuous, f)
+            else:
+                assert state.fpn_profile is not None and np.array_equal(state.fpn_profile, ref.profile), \
+                    (count, continuous, f)
+            if continuous and ref.profile is not None:
+                # a continuous profile never leaves the earliest window
+                first = ref.profile if first is None else first
+                assert np.array_equal(state.fpn_profile, first)
+
+
+def test_fpn_update_replays_the_reference_accumulation():
+    check_fpn_update(O.fpn_update)
+
+
+def _fpn_update_latest_window(state, X, p):
+    """Planted: determination over the most recent `required` A-scans."""
+    required = p.fpn_bscan_count * p.ascans_per_bscan
+    state.fpn_rows.extend(list(X))
+    state.fpn_rows = state.fpn_rows[-required:]
+    state.fpn_count += X.shape[0]
+    if not (state.fpn_requested or p.fpn_continuous) or state.fpn_count < required:
+        return
+    state.fpn_profile = O.min_variance_mean(np.asarray(state.fpn_rows), 8)[: 2 * (p.signal_length // 2)].copy()
+    if not p.fpn_continuous:
+        state.fpn_rows, state.fpn_count = [], 0
+    state.fpn_requested = False
+
+
+def _fpn_update_no_clear(state, X, p):
+    """Planted: the one-shot determination keeps its accumulation."""
+    keep_requested = state.fpn_requested
+    p2 = O.ChainParams(**{**p.__dict__, "fpn_continuous": True}) if keep_requested else p
+    O.fpn_update(state, X, p2)
+
+
+def _fpn_update_bscans_only(state, X, p):
+    """Planted: required = bscan count (the ascans-per-B-scan factor dropped)."""
+    p2 = O.ChainParams(**{**p.__dict__, "ascans_per_bscan": 1})
+    O.fpn_update(state, X, p2)
+
+
+@pytest.mark.parametrize("planted", [_fpn_update_latest_window, _fpn_update_no_clear, _fpn_update_bscans_only])
+def test_planted_fpn_update_mistakes_are_caught(planted):
+    with pytest.raises(AssertionError):
+        check_fpn_update(planted)
+
+
+# ------------------------------------------------------------------ post-process background
+
+def check_record_background(fn):
+    rng = np.random.default_rng(78)
+    b, a, w = 3, 8, 12
+    base = rng.integers(-50, 50, w).astype(np.float64)
+    # each column's deviations cancel exactly: +d and -d in paired rows
+    d = rng.integers(-20, 20, (b * a // 2, w)).astype(np.float64)
+    out = base[None, :] + np.concatenate([d, -d], 0)
+    out = out[rng.permutation(b * a)]
+    assert np.array_equal(fn(out), base)  # mean over every A-scan of every B-scan
+    # the reference's float accumulation (cpu_backend.cpp:105-118) on data
+    # that is not exactly representable agrees to float rounding
+    f = rng.normal(3.0, 1.0, (b * a, w)).astype(np.float32)
+    replay = np.zeros(w, dtype=np.float32)
+    for i in range(w):
+        s = np.float32(0.0)
+        for r in range(b * a):
+            s = np.float32(s + f[r, i])
+        replay[i] = np.float32(s / np.float32(b * a))
+    assert np.allclose(fn(f.astype(np.float64)), replay, rtol=1e-6, atol=0)
+
+
+def test_record_background_is_the_column_mean():
+    check_record_background(O.record_background)
+
+
+@pytest.mark.parametrize("planted", [
+    lambda out: out[:8].mean(axis=0),        # first B-scan only (what the CUDA backend records)
+    lambda out: np.median(out, axis=0),      # median instead of mean
+    lambda out: out.sum(axis=0) / (out.shape[0] - 1),
+])
+def test_planted_record_background_mistakes_are_caught(planted):
+    with pytest.raises(AssertionError):
+        check_record_background(planted)
+
+
+# ------------------------------------------------------------------ dispersion with a mismatched phasor
+
+def check_dispersion(fn):
+    rng = np.random.default_rng(79)
+    n = 32
+    y = rng.normal(0, 1, (3, n))
+    ang = rng.uniform(-np.pi, np.pi, n)
+    ph = np.empty(2 * n, np.float32)
+    ph[0::2], ph[1::2] = np.cos(ang), np.sin(ang)
+    c, s = ph[0::2].astype(np.float64), ph[1::2].astype(np.float64)
+    got = fn(y, ph)
+    for r in range(3):
+        for j in range(n):
+            assert got[r, j] == complex(y[r, j] * c[j], y[r, j] * s[j])
+    # any other length leaves the data real and unchanged (cpu_kernels.tpp:205-208)
+    for bad in (np.concatenate([ph, ph[:2]]), ph[:-2], np.zeros(0, np.float32)):
+        got = fn(y, bad)
+        assert np.array_equal(got.real, y) and not np.any(got.imag)
+
+
+def test_dispersion_length_rule():
+    check_dispersion(O.dispersion)
+
+
+def _dispersion_truncating(y, ph):
+    """Planted: a longer phasor is truncated to N instead of ignored."""
+    n = y.shape[-1]
+    p = np.asarray(ph, np.float32).astype(np.float64)
+    if p.size >= 2 * n:
+        p = p[: 2 * n]
+    return O.dispersion(y, p.astype(np.float32))
+
+
+def test_planted_dispersion_mistake_is_caught():
+    with pytest.raises(AssertionError):
+        check_dispersion(_dispersion_truncating)
+
+
+# ------------------------------------------------------------------ mean A-line subtraction
+
+def check_mean_subtraction(fn):
+    rng = np.random.default_rng(80)
+    n = 16
+    X = rng.normal(0, 1, (4, n)) + 1j * rng.normal(0, 1, (4, n))
+    for pairs in (0, 3, n // 2, n // 2 + 5, n):
+        m = rng.normal(0, 1, 2 * pairs).astype(np.float32)
+        got = fn(X, m if pairs else np.zeros(0, np.float32))
+        use = min(n // 2, pairs)
+        for k in range(n):
+            want = X[:, k] - (complex(m[2 * k], m[2 * k + 1]) if k < use else 0)
+            assert np.array_equal(got[:, k], want), (pairs, k)
+
+
+def test_mean_subtraction_touches_min_half_pairs():
+    check_mean_subtraction(O.mean_aline_subtraction)
+
+
+@pytest.mark.parametrize("planted", [
+    # every pair of the profile, up to N
+    lambda X, m: X - np.pad(m[0::2] + 1j * m[1::2], (0, max(0, X.shape[-1] - len(m) // 2)))[: X.shape[-1]]
+    if len(m) else X,
+    # the conjugate of the profile
+    lambda X, m: O.mean_aline_subtraction(X, np.stack([m[0::2], -m[1::2]], -1).reshape(-1)) if len(m) else X,
+])
+def test_planted_mean_subtraction_mistakes_are_caught(planted):
+    with pytest.raises(AssertionError):
+        check_mean_subtraction(planted)
+
+
+# ------------------------------------------------------------------ input conversion with bitshift
+
+def check_bitshift(fn):
+    v8 = np.arange(256, dtype=np.uint8)
+    assert np.array_equal(fn(v8, True, True), np.array([v // 16 for v in range(256)], np.float64))
+    v16 = np.arange(65536, dtype=np.uint32).astype(np.uint16)
+    assert np.array_equal(fn(v16, True, True), (np.arange(65536) // 16).astype(np.float64))
+    # 32-bit integers: x / 2^32 (cuda_kernels.cu:58-59), an exact scaling
+    v32 = np.array([0, 1, 15, 16, 2 ** 24 + 1, 2 ** 31, 2 ** 32 - 1, 123456789], dtype=np.uint32)
+    assert [float(x) for x in fn(v32, True, True)] == [math.ldexp(int(x), -32) for x in v32]
+    # IEEE floats are not shifted; without the option every width is exact
+    f = np.array([0.25, 1.5, 3e4], np.float32)
+    assert np.array_equal(fn(f, True, True), f.astype(np.float64))
+    assert np.array_equal(fn(v16, True, False), np.arange(65536, dtype=np.float64))
+    assert [float(x) for x in fn(v32, True, False)] == [float(int(x)) for x in v32]
+
+
+def test_bitshift_conversion():
+    check_bitshift(O.convert_input)
+
+
+@pytest.mark.parametrize("planted", [
+    lambda raw, ieee, shift: O.convert_input(raw, ieee, False) / 16.0 if shift else O.convert_input(raw, ieee, False),
+    lambda raw, ieee, shift: (raw.astype(np.uint64) >> (4 if shift else 0)).astype(np.float64)
+    if raw.dtype != np.float32 else raw.astype(np.float64),
+])
+def test_planted_bitshift_mistakes_are_caught(planted):
+    with pytest.raises(AssertionError):
+        check_bitshift(planted)
