@@ -35,11 +35,12 @@ public:
     size_t getSizeInBytes() const;
 
 private:
-    friend struct detail::IOBufferAccess;  // lets a backend adopt pinned memory
+    // lets a backend adopt pinned memory; adds no member, so the layout is
+    // the reference's three members
+    friend struct detail::IOBufferAccess;
     void* dataPtr;
     size_t sizeInBytes;
     DataType dataType;
-    void (*freeFn)(void*);
 };
 
 }  // namespace ope

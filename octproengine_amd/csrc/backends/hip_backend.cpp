@@ -173,11 +173,13 @@ void HipBackend::process(IOBuffer& input) {
     InputCtx* ic = nullptr;
     for (int i = 0; i < kInputBuffers && i < static_cast<int>(inputs_.size()); ++i)
         if (inputs_[i].get() == &input) ic = &inputCtx_[i];
+    // ope_hip_submit_host takes its frame slot in the same rotation and
+    // advances it only on success: both counters advance together
     SlotCtx& s = slots_[nextSlot_];
-    nextSlot_ = (nextSlot_ + 1) % kSlots;
     check(ope_hip_submit_host(ctx_, input.getDataPointer(), static_cast<float*>(s.output->getDataPointer()),
                               ic ? &HipBackend::onInputFree : nullptr, ic, &HipBackend::onFrameDone, &s),
           "process");
+    nextSlot_ = (nextSlot_ + 1) % kSlots;
 }
 
 void HipBackend::updateConfig(const ProcessorConfiguration& config) {

@@ -2,8 +2,14 @@
 // src/core/iobuffer.cpp:46-119 and src/core/types.cpp:5-53): 64-byte aligned,
 // zero-initialised host memory.  A backend may instead hand a buffer pinned
 // memory through detail::IOBufferAccess, with the matching release function.
+// IOBuffer keeps the reference's three members (include/iobuffer.h:30-33),
+// so the release functions of adopted memory live in a registry keyed by
+// the data pointer rather than in the object.
+#include <cstddef>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
+#include <unordered_map>
 
 #include "iobuffer.h"
 #include "iobuffer_access.h"
@@ -31,7 +37,18 @@ int getDataTypeBitDepth(DataType type) {
 
 int getDataTypeByteSize(DataType type) { return getDataTypeBitDepth(type) / 8; }
 
-IOBuffer::IOBuffer() : dataPtr(nullptr), sizeInBytes(0), dataType(DataType::UINT8), freeFn(nullptr) {}
+namespace {
+
+// Release functions of memory adopted through detail::IOBufferAccess.
+std::mutex g_adopted_mu;
+std::unordered_map<void*, void (*)(void*)>& adopted() {
+    static std::unordered_map<void*, void (*)(void*)> m;
+    return m;
+}
+
+}  // namespace
+
+IOBuffer::IOBuffer() : dataPtr(nullptr), sizeInBytes(0), dataType(DataType::UINT8) {}
 
 IOBuffer::~IOBuffer() { this->releaseMemory(); }
 
@@ -43,16 +60,23 @@ bool IOBuffer::allocateMemory(size_t bytes) {
     std::memset(p, 0, bytes);
     this->dataPtr = p;
     this->sizeInBytes = bytes;
-    this->freeFn = std::free;
     return true;
 }
 
 void IOBuffer::releaseMemory() {
     if (this->dataPtr) {
-        if (this->freeFn) this->freeFn(this->dataPtr);
+        void (*release)(void*) = std::free;
+        {
+            std::lock_guard<std::mutex> lk(g_adopted_mu);
+            auto it = adopted().find(this->dataPtr);
+            if (it != adopted().end()) {
+                release = it->second;
+                adopted().erase(it);
+            }
+        }
+        if (release) release(this->dataPtr);
         this->dataPtr = nullptr;
         this->sizeInBytes = 0;
-        this->freeFn = nullptr;
     }
 }
 
@@ -65,11 +89,27 @@ size_t IOBuffer::getSizeInBytes() const { return this->sizeInBytes; }
 
 namespace detail {
 
+// The reference's IOBuffer members (include/iobuffer.h:30-33), in order.
+struct ReferenceIOBufferLayout {
+    void* dataPtr;
+    size_t sizeInBytes;
+    DataType dataType;
+};
+
 void IOBufferAccess::adopt(IOBuffer& b, void* ptr, size_t bytes, void (*release)(void*)) {
+    static_assert(sizeof(IOBuffer) == sizeof(ReferenceIOBufferLayout), "IOBuffer size differs from the reference");
+    static_assert(alignof(IOBuffer) == alignof(ReferenceIOBufferLayout), "IOBuffer alignment differs");
+    static_assert(offsetof(IOBuffer, dataPtr) == offsetof(ReferenceIOBufferLayout, dataPtr), "dataPtr offset");
+    static_assert(offsetof(IOBuffer, sizeInBytes) == offsetof(ReferenceIOBufferLayout, sizeInBytes),
+                  "sizeInBytes offset");
+    static_assert(offsetof(IOBuffer, dataType) == offsetof(ReferenceIOBufferLayout, dataType), "dataType offset");
     b.releaseMemory();
+    if (ptr) {
+        std::lock_guard<std::mutex> lk(g_adopted_mu);
+        adopted()[ptr] = release;
+    }
     b.dataPtr = ptr;
     b.sizeInBytes = ptr ? bytes : 0;
-    b.freeFn = release;
 }
 
 }  // namespace detail

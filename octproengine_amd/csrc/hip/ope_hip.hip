@@ -1360,8 +1360,9 @@ int ope_hip_submit_host(ope_hip_ctx* c, const void* host_in, float* host_out, vo
     if (!host_in || !host_out) return fail(c, OPE_HIP_EINVAL, "null host buffer");
     std::lock_guard<std::mutex> lk(c->submit_mu);
     HIP_TRY(c, hipSetDevice(c->device));
+    // the slot advances only when the frame is enqueued, as the caller's
+    // output-buffer index does (HipBackend::process)
     const int si = c->next_slot;
-    c->next_slot = (c->next_slot + 1) % (int)c->slots.size();
     Slot& s = c->slots[si];
     if (s.used) HIP_TRY(c, hipEventSynchronize(s.d2h_done));  // slot buffers and callback record free
     s.used = true;
@@ -1383,6 +1384,7 @@ int ope_hip_submit_host(ope_hip_ctx* c, const void* host_in, float* host_out, vo
     HIP_TRY(c, hipMemcpyAsync(host_out, s.d_out, out_bytes, hipMemcpyDeviceToHost, c->s_out));
     HIP_TRY(c, hipLaunchHostFunc(c->s_out, host_done_cb, &hc));
     HIP_TRY(c, hipEventRecord(s.d2h_done, c->s_out));
+    c->next_slot = (si + 1) % (int)c->slots.size();
     return OPE_HIP_OK;
 }
 

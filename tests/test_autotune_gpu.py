@@ -18,18 +18,21 @@ pytestmark = pytest.mark.gpu
 PASSES = {256: 3, 512: 3, 1024: 4, 2048: 4, 4096: 4}
 
 
-def _chain(n, a, method, dc=False):
+def _chain(n, a, method, dc=False, window=32):
     p = db_params(O.ChainParams(signal_length=n, ascans_per_bscan=a, resampling=True, interpolation=method,
-                                windowing=True, dispersion=True, dc_removal=dc, dc_window=32))
+                                windowing=True, dispersion=True, dc_removal=dc, dc_window=window))
     t = standard_tables(n)
     return p, t
 
 
+# linear + DC (window 64, the reference default) runs the banded DC path in
+# every variant: two A-scans per lane with the staging sharing buffer B,
+# the Stockham placements, and N = 4096 (512 lanes per A-scan)
 @pytest.mark.parametrize("n", [256, 1024, 2048, 4096])
-@pytest.mark.parametrize("method,dc", [(O.CUBIC, False), (O.LANCZOS, True)])
+@pytest.mark.parametrize("method,dc", [(O.CUBIC, False), (O.LANCZOS, True), (O.LINEAR, True)])
 def test_every_launch_variant_meets_parity(n, method, dc):
     a = 96
-    p, t = _chain(n, a, method, dc)
+    p, t = _chain(n, a, method, dc, window=64 if method == O.LINEAR else 32)
     raw = S.fringes(n, a, seed=n + method)
     ref = O.process_frame(raw, p, t.curve, t.window, t.phasor)
     g = GpuChain(p, t.curve, t.window, t.phasor)
@@ -37,6 +40,7 @@ def test_every_launch_variant_meets_parity(n, method, dc):
     variants = [(1, k, 0) for k in range(PASSES[n] if small else 1)] + [(2, PASSES[n] - 1, 0), (1, PASSES[n] - 1, 1)]
     if n in (1024, 2048):
         variants.append((1, PASSES[n], 0))  # the swap schedule
+        variants.append((2, PASSES[n], 0))  # swap schedule, two A-scans per lane
     for v in variants:
         if not small and v[0] == 1 and v[1] != PASSES[n] - 1:
             continue

@@ -131,7 +131,7 @@ def test_dc_removal(dtype, window):
 @pytest.mark.parametrize("method", ["none", "linear"])
 @pytest.mark.parametrize("dtype", [np.uint8, np.uint16])
 @pytest.mark.parametrize("window", [0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 31, 56, 63, 64, 65])
-@pytest.mark.parametrize("n", [1024, 2048])
+@pytest.mark.parametrize("n", [1024, 2048, 4096])
 def test_dc_removal_banded(n, window, dtype, method):
     a = 23
     raw = _typed(S.fringes(n, a, seed=7 * n + window), dtype)

@@ -68,12 +68,14 @@ def oracle_for(ope, p, raw, interp, dc=False):
     return O.process_frame(raw, params, curve, window, phasor)
 
 
-@pytest.mark.parametrize("interp", ["LINEAR", "CUBIC", "LANCZOS"])
-def test_processor_single_frame_matches_oracle(ope, interp):
+# the Processor tunes its launch at initialisation: LINEAR + DC takes the
+# banded DC path in whichever variant the tuner picks
+@pytest.mark.parametrize("interp,dc", [("LINEAR", False), ("LINEAR", True), ("CUBIC", True), ("LANCZOS", False)])
+def test_processor_single_frame_matches_oracle(ope, interp, dc):
     n, a, b = 2048, 256, 2
     method = getattr(ope.InterpolationMethod, interp)
     p = ope.Processor(ope.Backend.CUDA)
-    configure(ope, p, n, a, b, method, dc=(interp == "CUBIC"))
+    configure(ope, p, n, a, b, method, dc=dc)
     p.initialize()
     col = Collector()
     p.set_callback(col)
@@ -84,9 +86,9 @@ def test_processor_single_frame_matches_oracle(ope, interp):
     p.process(buf)
     assert col.done.wait(30), "no output callback"
     out = col.outputs[0].reshape(a * b, n // 2)
-    ref = oracle_for(ope, p, raw, method, dc=(interp == "CUBIC"))
+    ref = oracle_for(ope, p, raw, method, dc=dc)
     e = db_error(out, ref)
-    print(f"Processor {interp}: dB error near peak {e:.2e}")
+    print(f"Processor {interp} dc={dc}: dB error near peak {e:.2e}")
     assert e <= DB_TOL
     p.stop()
 
