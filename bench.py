@@ -16,7 +16,12 @@ the stream the kernels run on.
 Printed: one JSON line with the throughput (A-scans/s), the roofline of the
 chain kernel (HBM-bound: algorithmic bytes = N*2 B in + N/2*4 B out per
 A-scan) and the CPU baseline (the float32 C restatement of the reference CPU
-backend in oracle/, timed on a bounded sample on this host).
+backend in oracle/, timed on a bounded sample on this host).  Beside them:
+four frames per launch, the mixed-length ragged batch, the Lanczos-8 + DC
+volume (graph and one launch), the autofocus step, the 2048 x 512 x 1 frame
+three ways (per-frame launches, a 64-frame graph, one launch over the ring),
+host frames through the PCIe (ope_hip_submit_host), and the reference's own
+benchmark configurations end to end through octproengine.Processor.
 """
 
 from __future__ import annotations
@@ -239,6 +244,120 @@ def volume_measurement(capi, S, torch, stream, reps=3):
                            "ms_per_volume": round(ms_one, 3)}}
 
 
+def c1_measurement(capi, S, torch, stream, ring=64, reps=20):
+    """The 2048 x 512 x 1 uint16 frame (linear resampling, Hann, log, no
+    dispersion: two real A-scans per complex FFT), where a frame is only
+    4 MiB and launch latency matters: a ring of `ring` such frames processed
+    one launch per frame, as one captured HIP graph of `ring` frames, and
+    as one persistent launch over the whole ring.  The three give identical
+    outputs (same kernel and variant)."""
+    n, a = 2048, 512
+    params = capi.make_params(signal_length=n, ascans_per_bscan=a, bscans_per_buffer=1, input_type=capi.UINT16,
+                              resampling_enabled=1, interpolation=capi.LINEAR, windowing_enabled=1,
+                              dispersion_enabled=0, log_scaling=1, grayscale_min=30.0, grayscale_max=100.0)
+    ctx = capi.Context(params)
+    ctx.set_resampling_curve(S.resampling_curve(n))
+    ctx.set_window_curve(hann_window(n))
+    vol = S.fringes_torch(n, a * ring, seed=91, device="cuda")
+    out = torch.empty((a * ring, n // 2), dtype=torch.float32, device="cuda")
+    sp = stream.cuda_stream
+    ins = [vol[f * a:(f + 1) * a].data_ptr() for f in range(ring)]
+    outs = [out[f * a:(f + 1) * a].data_ptr() for f in range(ring)]
+    ctx.process_device(ins[0], outs[0], a, sp)  # FPN statistics of the first B-scan
+    torch.cuda.synchronize()
+    variant, _ = ctx.autotune(ins[0], outs[0], a, sp)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+    def timed(fn, count):
+        fn()
+        torch.cuda.synchronize()
+        e0.record(stream)
+        for _ in range(count):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) * 1e3 / count  # microseconds per call
+
+    def per_frame():
+        for f in range(ring):
+            ctx.process_device(ins[f], outs[f], a, sp)
+
+    us_frames = timed(per_frame, reps) / ring
+    ref = out.clone()
+    graph = capi.FrameGraph(ctx, ins, outs, a)
+    us_graph = timed(lambda: graph.launch(sp), reps) / ring
+    same_graph = bool(torch.equal(out, ref))
+    us_one = timed(lambda: ctx.process_device(vol.data_ptr(), out.data_ptr(), a * ring, sp), reps) / ring
+    same_one = bool(torch.equal(out, ref))
+    graph.close()
+    ctx.close()
+    assert same_graph and same_one, "the three forms must give identical outputs"
+
+    def rate(us):
+        return {"value": round(a / (us / 1e6), 1), "unit": "A-scans/s", "us_per_frame": round(us, 2)}
+
+    return {"workload": f"2048 x 512 x 1 uint16, linear + Hann + log, ring of {ring} frames",
+            "launch_variant": variant, "one_launch_per_frame": rate(us_frames),
+            f"graph_{ring}_frames": rate(us_graph), "one_launch_over_ring": rate(us_one)}
+
+
+def processor_end_to_end(iterations=100):
+    """The reference's own benchmark (tests/test_performance_benchmark.cpp:
+    26-56,179-238) through octproengine.Processor(Backend.CUDA): cubic
+    {0.5, 2048, -100, 0}, Hann 0.5 / 0.95, dispersion {0, 0, 1, 2}, rolling-
+    average background removal, log 30..100.  Per configuration, wall clock
+    from the first get_next_available_buffer to the last output callback of
+    `iterations` frames, each frame copied into the pinned input buffer
+    first; A-scans/s = 1000 * B / avg_ms * A, MB/s counts input bytes."""
+    import threading
+
+    import numpy as np
+
+    from octproengine_amd import octproengine as ope
+    from octproengine_amd import synthetic as S
+    rows = []
+    for a, b in ((256, 1), (256, 8), (1024, 1), (1024, 8)):
+        n = 2048
+        p = ope.Processor(ope.Backend.CUDA)
+        p.set_input_parameters(n, a, b, ope.DataType.UINT16)
+        p.enable_resampling(True)
+        p.set_interpolation_method(ope.InterpolationMethod.CUBIC)
+        p.set_resampling_coefficients([0.5, 2048.0, -100.0, 0.0])
+        p.enable_windowing(True)
+        p.set_window_parameters(ope.WindowType.HANN, 0.5, 0.95)
+        p.enable_dispersion_compensation(True)
+        p.set_dispersion_coefficients([0.0, 0.0, 1.0, 2.0], 1.0)
+        p.enable_background_removal(True)
+        p.enable_log_scaling(True)
+        p.set_grayscale_range(30.0, 100.0)
+        p.enable_bscan_flip(False)
+        p.initialize()
+        data = S.fringes(n, a * b, seed=5).reshape(-1)
+        done = threading.Semaphore(0)
+        p.set_callback(lambda out: done.release())
+        for _ in range(4):  # warm-up frames
+            buf = p.get_next_available_buffer()
+            buf[:] = data
+            p.process(buf)
+        for _ in range(4):
+            done.acquire()
+        t0 = time.perf_counter()
+        for _ in range(iterations):
+            buf = p.get_next_available_buffer()
+            buf[:] = data
+            p.process(buf)
+        for _ in range(iterations):
+            done.acquire()
+        ms = (time.perf_counter() - t0) * 1e3 / iterations
+        p.stop()
+        rows.append({"signal_length": n, "ascans_per_bscan": a, "bscans_per_buffer": b, "avg_ms": round(ms, 3),
+                     "ascans_per_s": round(1000.0 * b / ms * a, 1),
+                     "mb_per_s": round(n * a * b * 2 * 1000.0 / ms / (1 << 20), 1)})
+    return {"workload": "reference benchmark configuration, 100 frames each, wall clock incl. the copy into the "
+                        "pinned input buffer, H2D, chain, D2H and the output callback",
+            "configs": rows}
+
+
 def autofocus_measurement(capi, S, torch):
     """Dispersion autofocus on 2048 x 512 x 8 float32 fringes made with
     d2 = 8e-6, d3 = -3e-9: time of one loss + gradient evaluation and of 50
@@ -351,15 +470,19 @@ def main():
     ap.add_argument("--no-ragged", action="store_true", help="skip the mixed-length batch measurement")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the volume (graph) and autofocus measurements")
-    ap.add_argument("--host-path", action="store_true",
-                    help="also time host frames through ope_hip_submit_host (pinned host buffers, "
-                         "H2D + chain + D2H): the PCIe-inclusive rate, reported beside the value")
+    ap.add_argument("--no-host-path", action="store_true",
+                    help="skip the host-frame timing through ope_hip_submit_host (pinned host buffers, "
+                         "H2D + chain + D2H: the PCIe-inclusive rate, reported beside the value) and "
+                         "the end-to-end Processor timing of the reference benchmark configurations")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 256 MiB read + 256 MiB write streaming kernel, the "
                          "reference point for the FETCH_SIZE / WRITE_SIZE counters")
     ap.add_argument("--sweep", action="store_true",
                     help="instead of the benchmark line, print one JSON line per chain configuration "
                          "(lengths, k-linearizations, dispersion, DC removal), each timed on its own")
+    ap.add_argument("--only", default=None, metavar="N,METHOD,DISP,DC",
+                    help="with --sweep: time this one configuration only, e.g. 2048,lanczos,1,1 "
+                         "(METHOD none|linear|cubic|lanczos), for profiler passes")
     ap.add_argument("--variant", default=None, metavar="AS,PASS,WG_PER_CU",
                     help="pin the launch variant (A-scans per lane, position of the small-radix FFT pass, "
                          "workgroups per CU) instead of autotuning it, e.g. 1,3,3 for profiler passes")
@@ -378,7 +501,11 @@ def main():
     from octproengine_amd import synthetic as S
 
     if args.sweep:
-        for c in sweep_configs():
+        cfgs = sweep_configs()
+        if args.only:
+            n_, m_, d_, dc_ = args.only.split(",")
+            cfgs = [(int(n_), m_, bool(int(d_)), bool(int(dc_)))]
+        for c in cfgs:
             print(json.dumps(sweep_one(capi, S, torch, *c)), flush=True)
         return
 
@@ -446,9 +573,11 @@ def main():
     ragged = None if args.no_ragged else ragged_measurement(capi, S, torch, stream)
     volume = None if args.no_extras else volume_measurement(capi, S, torch, stream)
     focus = None if args.no_extras else autofocus_measurement(capi, S, torch)
+    c1 = None if args.no_extras else c1_measurement(capi, S, torch, stream)
 
     host_rate = None
-    if args.host_path:
+    e2e = None
+    if not args.no_host_path:
         import threading
         in_bytes = frame * n * 2
         out_bytes = frame * (n // 2) * 4
@@ -476,6 +605,7 @@ def main():
         ctx.synchronize()
         for ptr in hin + hout:
             capi.lib.ope_hip_host_free(ctypes.c_void_p(ptr))
+        e2e = processor_end_to_end()
 
     if args.copy_calibration:
         src = torch.ones(64 << 20, dtype=torch.float32, device=dev)
@@ -517,9 +647,11 @@ def main():
         "ragged_batch": ragged,
         "volume": volume,
         "autofocus": focus,
+        "c1_latency_bound": c1,
         "host_buffers_pcie_inclusive": (None if host_rate is None else
                                         {"value": round(host_rate, 1), "unit": "A-scans/s",
                                          "path": "ope_hip_submit_host, 2 pinned frames in flight"}),
+        "processor_end_to_end": e2e,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(),
                      "kernel": f"chain_kernel<2048, 4, 0, 2, {variant['ascans_per_lane']}, {variant['small_radix_pass']}>", "bytes_per_launch": algo_bytes},
