@@ -195,14 +195,15 @@ def ragged_measurement(capi, S, torch, stream, reps=3):
     }
 
 
-def volume_measurement(capi, S, torch, stream, reps=3):
+def volume_measurement(capi, S, torch, stream, reps=3, method=None):
     """A 2048 x 1024 x 256 uint16 volume (8 reflectors per A-scan), Lanczos
-    resampling + Hann + dispersion + DC removal + log, as 16 frames of
-    16 B-scans: replayed from one captured HIP graph, and as one launch over
-    the whole volume."""
+    resampling (a = 8, the reference's LANCZOS, or a = 4) + Hann +
+    dispersion + DC removal + log, as 16 frames of 16 B-scans: replayed from
+    one captured HIP graph, and as one launch over the whole volume."""
+    method = capi.LANCZOS if method is None else method
     n, frame, frames = 2048, 16 * 1024, 16
     params = capi.make_params(signal_length=n, ascans_per_bscan=1024, bscans_per_buffer=16, input_type=capi.UINT16,
-                              resampling_enabled=1, interpolation=capi.LANCZOS, windowing_enabled=1,
+                              resampling_enabled=1, interpolation=method, windowing_enabled=1,
                               dispersion_enabled=1, dc_removal_enabled=1, dc_window_size=64, log_scaling=1,
                               grayscale_min=30.0, grayscale_max=100.0)
     ctx = capi.Context(params)
@@ -236,7 +237,8 @@ def volume_measurement(capi, S, torch, stream, reps=3):
     total = frame * frames
     graph.close()
     ctx.close()
-    return {"workload": "2048 x 1024 x 256 uint16 volume, 8 reflectors, Lanczos-8 + Hann + dispersion + DC + log",
+    name = "Lanczos-8" if method == capi.LANCZOS else "Lanczos-4"
+    return {"workload": f"2048 x 1024 x 256 uint16 volume, 8 reflectors, {name} + Hann + dispersion + DC + log",
             "launch_variant": variant,
             "graph_16_frames": {"value": round(total / (ms_graph / 1e3), 1), "unit": "A-scans/s",
                                 "ms_per_volume": round(ms_graph, 3)},
@@ -398,7 +400,7 @@ def autofocus_measurement(capi, S, torch):
             "d2_d3_found": [float(f"{th[0]:.4e}"), float(f"{th[1]:.4e}")]}
 
 
-SWEEP_METHODS = {"none": -1, "linear": 0, "cubic": 1, "lanczos": 2}
+SWEEP_METHODS = {"none": -1, "linear": 0, "cubic": 1, "lanczos": 2, "lanczos4": 3}
 
 
 def sweep_configs():
@@ -410,6 +412,7 @@ def sweep_configs():
     A-scans at N = 2048, as the benchmark workload)."""
     cfgs = [(n, m, True, False) for n in (1024, 2048, 4096) for m in ("none", "linear", "cubic", "lanczos")]
     cfgs += [(n, m, False, False) for n in (1024, 2048, 4096) for m in ("linear", "cubic")]
+    cfgs += [(n, "lanczos4", True, dc) for n in (1024, 2048, 4096) for dc in (False, True)]
     cfgs += [(2048, "cubic", True, True), (2048, "lanczos", True, True),
              (1024, "none", True, True), (2048, "none", True, True),
              (256, "cubic", True, False), (512, "cubic", True, False)]
@@ -572,6 +575,7 @@ def main():
 
     ragged = None if args.no_ragged else ragged_measurement(capi, S, torch, stream)
     volume = None if args.no_extras else volume_measurement(capi, S, torch, stream)
+    volume4 = None if args.no_extras else volume_measurement(capi, S, torch, stream, method=capi.LANCZOS4)
     focus = None if args.no_extras else autofocus_measurement(capi, S, torch)
     c1 = None if args.no_extras else c1_measurement(capi, S, torch, stream)
 
@@ -646,6 +650,7 @@ def main():
         "batched_4_frames_per_launch": {"value": round(batched, 1), "unit": "A-scans/s"},
         "ragged_batch": ragged,
         "volume": volume,
+        "volume_lanczos4": volume4,
         "autofocus": focus,
         "c1_latency_bound": c1,
         "host_buffers_pcie_inclusive": (None if host_rate is None else

@@ -59,6 +59,11 @@ extern "C" {
 #define OPE_HIP_INTERP_LINEAR 0
 #define OPE_HIP_INTERP_CUBIC 1
 #define OPE_HIP_INTERP_LANCZOS 2
+/* Lanczos with a = 4 (8 taps, n0-3 .. n0+4), the kernel of
+ * cpu_kernels.tpp:148-162 at the other window size it is written for.  No
+ * counterpart in ope::InterpolationMethod, whose LANCZOS is a = 8
+ * (cpu_kernels.tpp:174): reachable through this C ABI only. */
+#define OPE_HIP_INTERP_LANCZOS4 3
 
 /* Plain-old-data snapshot of the ProcessorConfiguration fields the chain
  * reads (include/processorconfiguration.h:33-172; read by

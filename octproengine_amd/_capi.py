@@ -25,6 +25,7 @@ DTYPE_CODES = {np.dtype(np.uint8): UINT8, np.dtype(np.uint16): UINT16,
                np.dtype(np.int16): INT16, np.dtype(np.int32): INT32,
                np.dtype(np.float32): FLOAT32}
 LINEAR, CUBIC, LANCZOS = 0, 1, 2
+LANCZOS4 = 3  # C ABI only: Lanczos with a = 4 (8 taps)
 # sample offset modes (ope_hip_set_sample_offset)
 OFFSET_OFF, OFFSET_AUTO, OFFSET_FIXED = 0, 1, 2
 

@@ -37,7 +37,7 @@ HIP_FLAGS = [
     "-Wno-pass-failed",
 ]
 CHAIN_LENGTHS = (256, 512, 1024, 2048, 4096)
-RAGGED_TAPS = (1, 2, 4, 16)
+RAGGED_TAPS = (1, 2, 4, 8, 16)
 
 LIB_HIP = PKG / "libope_hip.so"
 LIB_CORE = PKG / "libope.so"

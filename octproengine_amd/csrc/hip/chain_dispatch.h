@@ -17,7 +17,7 @@ struct ChainLaunch {
     int slots;   // A-scans per workgroup iteration
 };
 
-// taps: 1, 2, 4 or 16; mode: M_DC / M_STATS bits; iw: input sample width in
+// taps: 1, 2, 4, 8 or 16; mode: M_DC / M_STATS bits; iw: input sample width in
 // bytes (1, 2, 4); AS: A-scans per lane (1, or 2 for the modes without
 // statistics); sched: position of the small-radix FFT pass (0 ..
 // passes - 1; other values: the last pass).  fn is null for a combination
@@ -29,7 +29,7 @@ ChainLaunch pick_chain(int taps, int mode, int iw, int sched);
 using RaggedFn = void (*)(RaggedArgs);
 
 // Ragged-batch kernel for taps T, mode (0 or M_DC) and input width iw.
-// Explicitly instantiated for T = 1, 2, 4, 16 (ragged_instances.hip
+// Explicitly instantiated for T = 1, 2, 4, 8, 16 (ragged_instances.hip
 // compiled with -DOPE_RAGGED_T=T).
 template <int T>
 RaggedFn pick_ragged_t(int mode, int iw);

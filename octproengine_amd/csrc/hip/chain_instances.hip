@@ -1,6 +1,6 @@
 // Chain-kernel instantiations of one signal length and one number of
 // A-scans per lane, OPE_CHAIN_N and OPE_CHAIN_AS (set on the compiler line).
-//  AS = 1: taps {1, 2, 4, 16} x modes {plain, DC} x input widths {1, 2, 4}
+//  AS = 1: taps {1, 2, 4, 8, 16} x modes {plain, DC} x input widths {1, 2, 4}
 //          bytes x every radix placement, plus the FPN-statistics modes with
 //          the default placement;
 //  AS = 2: taps x modes {plain, DC, pairs} x widths, default placement;
@@ -20,6 +20,7 @@ KernelFn pick_taps(int taps) {
         case 1: return chain_kernel<N, 1, MODE, IW, AS, SCHED>;
         case 2: return chain_kernel<N, 2, MODE, IW, AS, SCHED>;
         case 4: return chain_kernel<N, 4, MODE, IW, AS, SCHED>;
+        case 8: return chain_kernel<N, 8, MODE, IW, AS, SCHED>;
         default: return chain_kernel<N, 16, MODE, IW, AS, SCHED>;
     }
 }

@@ -979,7 +979,7 @@ struct ChainLds {
 // workgroup iteration taking SLOTS consecutive A-scans.
 //
 // T = taps per output sample: 1 (no resampling), 2 (linear), 4 (cubic),
-// 16 (Lanczos-8).  MODE: M_* bits.  IW: input sample width in bytes (1, 2,
+// 8 (Lanczos-4), 16 (Lanczos-8).  MODE: M_* bits.  IW: input sample width in bytes (1, 2,
 // 4); DC removal sums 1- and 2-byte samples in int, 4-byte ones in double.
 // AS: A-scans each lane carries through the chain at once (instruction-level
 // parallelism: the LDS traffic of one overlaps the arithmetic of the other,
@@ -1370,11 +1370,12 @@ __device__ __forceinline__ void chain_body(const ChainArgs A, float2* bufA, floa
 // spills; three is asked for wherever it needs no spills (every
 // throughput-mode kernel up to 4 taps, and DC removal of 8/16-bit samples).
 // Lanczos (T = 16) holds 128 tap weights per lane: two waves per SIMD with a
-// few spills measured faster than the one the compiler picks.  The FPN
+// few spills measured faster than the one the compiler picks; Lanczos-4
+// (T = 8, 64 weights) does not fit three without spills either.  The FPN
 // statistics modes and the double-precision DC prologue keep the default.
 template <int T, int MODE, int IW>
 constexpr int chain_waves_per_eu() {
-    if (T == 16) return 2;
+    if (T >= 8) return 2;
     if ((MODE & M_STATS) || (IW == 4 && (MODE & M_DC))) return OPE_WAVES_PER_EU;
     return 3;
 }

@@ -243,6 +243,7 @@ int taps_for(const ope_hip_params& p) {
     switch (p.interpolation) {
         case OPE_HIP_INTERP_LINEAR: return 2;
         case OPE_HIP_INTERP_CUBIC: return 4;
+        case OPE_HIP_INTERP_LANCZOS4: return 8;
         default: return 16;
     }
 }
@@ -250,12 +251,12 @@ int taps_for(const ope_hip_params& p) {
 // C static_cast<int>(float): truncation toward zero.
 inline int trunc_int(float v) { return static_cast<int>(v); }
 
-double lanczos8(double d) {
-    // cpu_kernels.tpp:148-162 evaluated in double
+// cpu_kernels.tpp:148-162 evaluated in double, window size a
+double lanczos(double d, int a) {
     if (std::fabs(d) < 1e-5) return 1.0;
-    if (std::fabs(d) >= 8.0) return 0.0;
+    if (std::fabs(d) >= (double)a) return 0.0;
     const double pd = M_PI * d;
-    const double pda = pd / 8.0;
+    const double pda = pd / (double)a;
     return (std::sin(pd) / pd) * (std::sin(pda) / pda);
 }
 
@@ -305,13 +306,15 @@ int build_taps_for(ope_hip_ctx* c, int n, int count, int T, const float* curve, 
                 idx[3] = n3; cw[3] = 0.5 * t3 - 0.5 * t2;
                 cnt = 4;
             } else {
+                // Lanczos, a = T/2: taps n0-a+1 .. n0+a (cpu_kernels.tpp:164-196)
+                const int a = T / 2;
                 const int n0 = trunc_int(r);
-                for (int i = -7; i <= 8; ++i) {
+                for (int i = -a + 1; i <= a; ++i) {
                     int k = n0 + i;
                     if (k < 0) k = 0;
                     if (k >= n) k = n - 1;
                     idx[cnt] = k;
-                    cw[cnt] = lanczos8((double)r - (double)k);
+                    cw[cnt] = lanczos((double)r - (double)k, a);
                     ++cnt;
                 }
             }
@@ -459,7 +462,7 @@ int validate_params(ope_hip_ctx* c, const ope_hip_params* p) {
         return fail(c, OPE_HIP_EINVAL, "ascans_per_bscan and bscans_per_buffer must be > 0");
     if (!input_class(p->input_type))
         return fail(c, OPE_HIP_EUNSUP, "input type %d not supported", p->input_type);
-    if (p->resampling_enabled && (p->interpolation < 0 || p->interpolation > 2))
+    if (p->resampling_enabled && (p->interpolation < 0 || p->interpolation > OPE_HIP_INTERP_LANCZOS4))
         return fail(c, OPE_HIP_EINVAL, "unknown interpolation method %d", p->interpolation);
     if (p->dc_removal_enabled && p->dc_window_size < 0)
         return fail(c, OPE_HIP_EINVAL, "rolling-average window size must be >= 0");
@@ -969,6 +972,7 @@ int build_ragged(ope_hip_ragged* r, ope_hip_ctx* const* ctxs, int n_ctx, const o
         case 1: fn = pick_ragged_t<1>(mode, (int)iw); break;
         case 2: fn = pick_ragged_t<2>(mode, (int)iw); break;
         case 4: fn = pick_ragged_t<4>(mode, (int)iw); break;
+        case 8: fn = pick_ragged_t<8>(mode, (int)iw); break;
         default: fn = pick_ragged_t<16>(mode, (int)iw); break;
     }
     r->fn = fn;
@@ -1487,7 +1491,11 @@ int ope_hip_op_convert_input(ope_hip_ctx* c, const void* input, int input_type, 
 int ope_hip_op_klinearization(ope_hip_ctx* c, const float* input, const float* curve, int method, int line_width,
                               int samples, float* out) {
     if (!c || !input || !curve || !out || line_width < 1 || samples < 0 || samples > line_width) return OPE_HIP_EINVAL;
-    const int T = method == OPE_HIP_INTERP_LINEAR ? 2 : method == OPE_HIP_INTERP_CUBIC ? 4 : method == OPE_HIP_INTERP_LANCZOS ? 16 : 0;
+    const int T = method == OPE_HIP_INTERP_LINEAR    ? 2
+                  : method == OPE_HIP_INTERP_CUBIC   ? 4
+                  : method == OPE_HIP_INTERP_LANCZOS ? 16
+                  : method == OPE_HIP_INTERP_LANCZOS4 ? 8
+                                                     : 0;
     if (!T) return fail(c, OPE_HIP_EINVAL, "unknown interpolation method %d", method);
     if (line_width < T) return fail(c, OPE_HIP_EINVAL, "line shorter than the interpolation kernel");
     if (samples == 0) return OPE_HIP_OK;

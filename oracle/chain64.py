@@ -36,6 +36,9 @@ import numpy as np
 # Interpolation / window enums, same values as the reference
 # (include/processorconfiguration.h:17-31).
 LINEAR, CUBIC, LANCZOS = 0, 1, 2
+# Lanczos with a = 4: the reference's kernel (cpu_kernels.tpp:148-162) at the
+# other window size it takes; its kLinearizationLanczos fixes a = 8 (:174)
+LANCZOS4 = 3
 HANN, GAUSS, SINE, LANCZOS_WIN, RECTANGULAR, FLAT_TOP = 0, 1, 2, 3, 4, 5
 
 FPN_SEGMENTS = 8  # src/backends/cpu/cpu_backend.cpp:306
@@ -146,7 +149,8 @@ def lanczos_kernel(d: np.ndarray, a: int = 8) -> np.ndarray:
 
 
 def klin_lanczos(x: np.ndarray, curve: np.ndarray, a: int = 8) -> np.ndarray:
-    """Lanczos-8 k-linearization, 16 taps i = -7..8 (cpu_kernels.tpp:164-196)."""
+    """Lanczos-a k-linearization, 2a taps i = -a+1..a (cpu_kernels.tpp:164-196;
+    the reference uses a = 8)."""
     n = x.shape[-1]
     r = np.asarray(curve, dtype=np.float32).astype(np.float64)
     n0 = _trunc_int(r)
@@ -328,6 +332,8 @@ def spectra(raw: np.ndarray, p: ChainParams, curve, window, phasor, ieee_float32
             y = klin_linear(x, curve)
         elif p.interpolation == CUBIC:
             y = klin_cubic(x, curve)
+        elif p.interpolation == LANCZOS4:
+            y = klin_lanczos(x, curve, a=4)
         else:
             y = klin_lanczos(x, curve)
     else:

@@ -42,7 +42,7 @@ typedef struct {
     int32_t n_ascans;
     int32_t dtype;
     int32_t resampling;
-    int32_t interpolation; /* 0 linear, 1 cubic, 2 lanczos */
+    int32_t interpolation; /* 0 linear, 1 cubic, 2 lanczos (a = 8), 3 lanczos (a = 4) */
     int32_t windowing;
     int32_t dispersion;    /* applied only if phasor_len == 2*N */
     int32_t dc_removal;
@@ -163,12 +163,12 @@ static float cubic_hermite(float y0, float y1, float y2, float y3, float pos) {
     return 0.5f * pos * (a * pos2 + b * pos + c) + y1;
 }
 
-static float lanczos8(float x) {
+static float lanczos(float x, int a) {
     if (fabsf(x) < 1e-5f) return 1.0f;
-    if (fabsf(x) >= 8.0f) return 0.0f;
+    if (fabsf(x) >= (float)a) return 0.0f;
     const float pi = (float)3.14159265358979323846;
     float px = pi * x;
-    float pxa = px / 8.0f;
+    float pxa = px / (float)a;
     return (sinf(px) / px) * (sinf(pxa) / pxa);
 }
 
@@ -226,13 +226,15 @@ static void chain_one(const ope_cpu_params *P, const fft_plan *plan, const void 
                 n2 = imin(n2, n - 1); n3 = imin(n3, n - 1);
                 v = cubic_hermite(x[n0], x[n1], x[n2], x[n3], nx - n1);
             } else {
+                /* Lanczos: a = 8 (method 2, the reference's), a = 4 (method 3) */
+                const int a = P->interpolation == 3 ? 4 : 8;
                 int n0 = (int)nx;
                 float sum = 0.0f;
-                for (int i = -7; i <= 8; ++i) {
+                for (int i = -a + 1; i <= a; ++i) {
                     int idx = n0 + i;
                     if (idx < 0) idx = 0;
                     if (idx >= n) idx = n - 1;
-                    sum += x[idx] * lanczos8(nx - idx);
+                    sum += x[idx] * lanczos(nx - idx, a);
                 }
                 v = sum;
             }

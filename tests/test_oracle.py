@@ -114,17 +114,17 @@ def test_planted_cubic_dropped_term_is_caught():
         check_cubic(bad)
 
 
-def check_lanczos(fn):
-    # sinc(d) sinc(d/8) in closed form, brute force
-    d = np.array([0.0, 1e-6, 0.3, 1.0, 2.5, 7.9, 8.0, 9.0, -3.2])
+def check_lanczos(fn, a=8):
+    # sinc(d) sinc(d/a) in closed form, brute force
+    d = np.array([0.0, 1e-6, 0.3, 1.0, 2.5, 3.9, 4.0, 4.5, 7.9, 8.0, 9.0, -3.2])
     want = []
     for v in d:
         if abs(v) < 1e-5:
             want.append(1.0)
-        elif abs(v) >= 8:
+        elif abs(v) >= a:
             want.append(0.0)
         else:
-            want.append(math.sin(math.pi * v) / (math.pi * v) * math.sin(math.pi * v / 8) / (math.pi * v / 8))
+            want.append(math.sin(math.pi * v) / (math.pi * v) * math.sin(math.pi * v / a) / (math.pi * v / a))
     assert np.allclose(fn(d), want, atol=1e-15)
 
 
@@ -142,6 +142,37 @@ def test_lanczos():
 def test_planted_lanczos_wrong_scale_is_caught():
     with pytest.raises(AssertionError):
         check_lanczos(lambda d: O.lanczos_kernel(np.asarray(d) * 1.01))
+
+
+def test_lanczos4():
+    """a = 4 (the extra 8-tap mode): closed form, interpolating at integers,
+    a constant stays nearly constant, and the taps are n0-3 .. n0+4: a
+    sample 4 or more positions away never enters."""
+    check_lanczos(lambda d: O.lanczos_kernel(d, 4), a=4)
+    y = RNG.normal(size=64)
+    ints = np.arange(8, 56, dtype=np.float32)
+    assert np.allclose(O.klin_lanczos(y, ints, a=4), y[8:56], atol=1e-12)
+    got = O.klin_lanczos(np.ones(64), np.linspace(10, 50, 77).astype(np.float32), a=4)
+    assert np.all(np.abs(got - 1.0) < 0.05)
+    for r in (20.0, 20.25, 20.75):
+        for far in (16, 25):
+            x = np.zeros(64)
+            x[far] = 1.0
+            assert O.klin_lanczos(x, np.array([r], np.float32), a=4)[0] == 0.0
+        # brute force over the 8 taps n0-3 .. n0+4 of the closed form
+        x = RNG.normal(size=64)
+        n0 = int(r)
+        want = sum(x[n0 + i] * (1.0 if abs(r - n0 - i) < 1e-5 else
+                                math.sin(math.pi * (r - n0 - i)) / (math.pi * (r - n0 - i))
+                                * math.sin(math.pi * (r - n0 - i) / 4) / (math.pi * (r - n0 - i) / 4))
+                   for i in range(-3, 5))
+        assert math.isclose(O.klin_lanczos(x, np.array([r], np.float32), a=4)[0], want, rel_tol=1e-12,
+                            abs_tol=1e-12)
+
+
+def test_planted_lanczos4_with_the_a8_window_is_caught():
+    with pytest.raises(AssertionError):
+        check_lanczos(lambda d: O.lanczos_kernel(d, 8), a=4)
 
 
 # ------------------------------------------------------------------ DC removal, FPN, scaling
@@ -243,7 +274,7 @@ def test_oracle_curves_match_reference_generator():
 
 # ------------------------------------------------------------------ C restatement vs FP64
 
-@pytest.mark.parametrize("method", [O.LINEAR, O.CUBIC, O.LANCZOS])
+@pytest.mark.parametrize("method", [O.LINEAR, O.CUBIC, O.LANCZOS, O.LANCZOS4])
 def test_cpu32_restatement_matches_fp64_oracle(method):
     from octproengine_amd import synthetic as S
     n, a = 512, 12
