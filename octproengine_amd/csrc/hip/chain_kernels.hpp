@@ -521,7 +521,7 @@ __device__ __forceinline__ void stockham_inverse(cf (&v)[AS][8], float2* const (
 // the swap schedule is launch variant "small-radix pass = NPASS"
 template <int N, int SCHED>
 constexpr bool swap_sched() {
-    return (N == 1024 || N == 2048) && SCHED == Geo<N>::NPASS;
+    return (N == 1024 || N == 2048 || N == 4096) && SCHED == Geo<N>::NPASS;
 }
 
 template <int N>
@@ -553,12 +553,69 @@ struct SwapFft {
     static constexpr int e3_write_off(int e) { return 8 * (e % RQ) + 64 * (e / RQ); }
 };
 
+// ------------------------------------------------------------------ N = 4096 with a register exchange
+//
+// Radices 8, 8, 8, 8 with n = 512 d0 + 64 d1 + 8 d2 + r2 and
+// k = k0 + 8 k1 + 64 k2 + 512 k3 (lane t of 512, one A-scan per workgroup):
+//   pass 0: DFT-8 over d0 (registers, lane t = r0 = 64 d1 + 8 d2 + r2),
+//     twiddle w_N^(r0 k0);
+//   exchange 1 through LDS at (t & 63) + 64 k0 + 512 (t >> 6), read at
+//     t + 512 d1: registers d1, lane t = r2 + 8 d2 + 64 k0, so d2 sits in
+//     lane bits 3-5 of every wavefront;
+//   pass 1: DFT-8 over d1, twiddle w_512^(r1 k1), r1 = t & 63;
+//   exchange 2 inside the wavefront: register bits 2, 1, 0 trade places
+//     with lane bits 5, 4, 3 (v_permlane32_swap, v_permlane16_swap, and a
+//     DPP row rotation by 8 with a select): registers d2, lane bits 3-5 k1;
+//   pass 2: DFT-8 over d2, twiddle w_64^(r2 k2), r2 = t & 7;
+//   exchange 3 through LDS at k0 + 8 k1 + 64 k2 + 513 r2, read at
+//     t + 513 r2: registers r2, lane t = k0 + 8 k1 + 64 k2;
+//   pass 3: DFT-8 over r2: v[e] = X[t + 512 e], the layout stockham_inverse
+//     leaves.
+// Every 32-lane half of each 8-byte LDS access hits 32 distinct bank pairs
+// (consecutive addresses; the odd stride 513 spreads the exchange-3
+// writes), without padding; the highest address, 4102, fits the buffer.
+struct Swap4096 {
+    static constexpr int E3 = 513;
+    __device__ static int e1_write_base(int t) { return (t & 63) + 512 * (t >> 6); }
+    __device__ static int e3_write_base(int t) { return (t >> 6) + 8 * ((t >> 3) & 7) + E3 * (t & 7); }
+};
+static_assert(7 + 8 * 7 + 64 * 7 + Swap4096::E3 * 7 < Geo<4096>::PADN, "exchange 3 fits the buffer");
+
+// Register bit 0 of the pair (a, b) trades places with lane bit 3: lanes
+// whose bit 3 is set take b from lane t ^ 8 into a, the others a from
+// lane t ^ 8 into b (DPP row_ror:8, which within a row of 16 lanes reads
+// lane t ^ 8).
+__device__ __forceinline__ void swap_lane_bit3(cf& a, cf& b) {
+    const bool hi = (threadIdx.x & 8) != 0;
+    const float pax = __int_as_float(dpp<0x128, 0xf>(__float_as_int(a.x)));
+    const float pay = __int_as_float(dpp<0x128, 0xf>(__float_as_int(a.y)));
+    const float pbx = __int_as_float(dpp<0x128, 0xf>(__float_as_int(b.x)));
+    const float pby = __int_as_float(dpp<0x128, 0xf>(__float_as_int(b.y)));
+    const cf na = {hi ? pbx : a.x, hi ? pby : a.y};
+    const cf nb = {hi ? b.x : pax, hi ? b.y : pay};
+    a = na;
+    b = nb;
+}
+
 // Per-lane twiddles of the swap schedule: tw[0][e] = w_N^(t e) (pass 0, lane
 // t = r0), tw[1][e] = w_TPA^(r1 e) (pass 1), tw[2][e] = w_(8S)^(r2 (e / RQ))
-// (pass 2); table = exp(+2 pi i k / N).
+// (pass 2); for N = 4096 tw[1][e] = w_512^((t & 63) e), tw[2][e] =
+// w_64^((t & 7) e); table = exp(+2 pi i k / N).
 template <int N>
 __device__ __forceinline__ void load_twiddles_swap(const float2* table, int t, cf (&tw)[Geo<N>::NPASS][8]) {
-    using F = SwapFft<N>;
+    if constexpr (N == 4096) {
+#pragma unroll
+        for (int e = 1; e < 8; ++e) {
+            float2 w = gload2(table, (t * e) & (N - 1));
+            tw[0][e] = {w.x, w.y};
+            w = gload2(table, (8 * (t & 63) * e) & (N - 1));
+            tw[1][e] = {w.x, w.y};
+            w = gload2(table, (64 * (t & 7) * e) & (N - 1));
+            tw[2][e] = {w.x, w.y};
+        }
+        return;
+    }
+    using F = SwapFft<N < 4096 ? N : 2048>;
     const int r1 = 8 * F::d2_of(t) + F::r2_of(t);
     const int r2 = F::r2_of(t);
 #pragma unroll
@@ -588,14 +645,74 @@ __device__ __forceinline__ void swap_lane_bit4(cf& a, cf& b) {
     b = {__uint_as_float(x[1]), __uint_as_float(y[1])};
 }
 
-// Unnormalised inverse FFT of the AS sequences whose elements t + e*N/8
-// lane t holds in v[u][e], by the swap schedule; on return v[u][e] =
-// X[t + e*N/8] as after stockham_inverse.  Exchange 1 goes through pa,
-// exchange 3 through pb; ends without a barrier (other wavefronts may still
-// read pb).  Every lane of each wavefront must be active.
-template <int N, int AS, typename Hook = NoHook>
-__device__ __forceinline__ void fft_swap(cf (&v)[AS][8], float2* const (&pa)[AS], float2* const (&pb)[AS],
-                                         const cf (&tw)[Geo<N>::NPASS][8], Hook&& after_last_reads = {}) {
+// The swap schedule of N = 4096 (see Swap4096).
+template <int AS, typename Hook>
+__device__ __forceinline__ void fft_swap4096(cf (&v)[AS][8], float2* const (&pa)[AS], float2* const (&pb)[AS],
+                                             const cf (&tw)[4][8], Hook&& after_last_reads) {
+    constexpr int TPA = 512;
+    const int t = threadIdx.x % TPA;
+    // pass 0 and exchange 1
+    const int w1 = Swap4096::e1_write_base(t);
+#pragma unroll
+    for (int u = 0; u < AS; ++u) {
+        idft8(v[u]);
+#pragma unroll
+        for (int e = 1; e < 8; ++e) v[u][e] = cmul(v[u][e], tw[0][e]);
+#pragma unroll
+        for (int e = 0; e < 8; ++e) pa[u][w1 + 64 * e] = make_float2(v[u][e].x, v[u][e].y);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < AS; ++u) {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const float2 w = pa[u][t + 512 * e];
+            v[u][e] = {w.x, w.y};
+        }
+    }
+    // pass 1, exchange 2 in registers, pass 2
+#pragma unroll
+    for (int u = 0; u < AS; ++u) {
+        idft8(v[u]);
+#pragma unroll
+        for (int e = 1; e < 8; ++e) v[u][e] = cmul(v[u][e], tw[1][e]);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) swap_lane_bit5(v[u][e], v[u][e + 4]);
+        swap_lane_bit4(v[u][0], v[u][2]);
+        swap_lane_bit4(v[u][1], v[u][3]);
+        swap_lane_bit4(v[u][4], v[u][6]);
+        swap_lane_bit4(v[u][5], v[u][7]);
+#pragma unroll
+        for (int e = 0; e < 8; e += 2) swap_lane_bit3(v[u][e], v[u][e + 1]);
+        idft8(v[u]);
+#pragma unroll
+        for (int e = 1; e < 8; ++e) v[u][e] = cmul(v[u][e], tw[2][e]);
+    }
+    // exchange 3 and pass 3
+    const int w3 = Swap4096::e3_write_base(t);
+#pragma unroll
+    for (int u = 0; u < AS; ++u) {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) pb[u][w3 + 64 * e] = make_float2(v[u][e].x, v[u][e].y);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < AS; ++u) {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const float2 w = pb[u][t + Swap4096::E3 * e];
+            v[u][e] = {w.x, w.y};
+        }
+    }
+    after_last_reads();
+#pragma unroll
+    for (int u = 0; u < AS; ++u) idft8(v[u]);
+}
+
+// The swap schedule of N = 1024 and 2048 (see SwapFft).
+template <int N, int AS, typename Hook>
+__device__ __forceinline__ void fft_swap_small(cf (&v)[AS][8], float2* const (&pa)[AS], float2* const (&pb)[AS],
+                                               const cf (&tw)[Geo<N>::NPASS][8], Hook&& after_last_reads) {
     using F = SwapFft<N>;
     const int t = threadIdx.x % F::TPA;
     // pass 0 and exchange 1
@@ -664,6 +781,21 @@ __device__ __forceinline__ void fft_swap(cf (&v)[AS][8], float2* const (&pa)[AS]
     after_last_reads();
 #pragma unroll
     for (int u = 0; u < AS; ++u) idft8(v[u]);
+}
+
+// Unnormalised inverse FFT of the AS sequences whose elements t + e*N/8
+// lane t holds in v[u][e], by the swap schedule; on return v[u][e] =
+// X[t + e*N/8] as after stockham_inverse.  Exchange 1 goes through pa,
+// exchange 3 through pb; ends without a barrier (other wavefronts may still
+// read pb).  Every lane of each wavefront must be active.
+template <int N, int AS, typename Hook = NoHook>
+__device__ __forceinline__ void fft_swap(cf (&v)[AS][8], float2* const (&pa)[AS], float2* const (&pb)[AS],
+                                         const cf (&tw)[Geo<N>::NPASS][8], Hook&& after_last_reads = {}) {
+    if constexpr (N == 4096) {
+        fft_swap4096<AS>(v, pa, pb, tw, after_last_reads);
+    } else {
+        fft_swap_small<N, AS>(v, pa, pb, tw, after_last_reads);
+    }
 }
 
 // Compile-time modes of the chain kernel.

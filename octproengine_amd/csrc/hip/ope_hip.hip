@@ -1835,7 +1835,7 @@ int ope_hip_autotune(ope_hip_ctx* c, const void* d_in, float* d_out, int64_t n_a
         if (as == 1 && small)
             for (int sc = 0; sc < np - 1; ++sc) scheds.push_back(sc);
         scheds.push_back(-1);
-        if (small) scheds.push_back(np);
+        if (small || c->n == 4096) scheds.push_back(np);
         for (const int sched : scheds) {
             const ChainLaunch l = pick_kernel(c->n, c->taps, variant_mode(a, mode, as), iw, as, sched);
             if (!l.fn) continue;

@@ -38,11 +38,11 @@ def test_every_launch_variant_meets_parity(n, method, dc):
     g = GpuChain(p, t.curve, t.window, t.phasor)
     small = n not in (512, 4096)
     variants = [(1, k, 0) for k in range(PASSES[n] if small else 1)] + [(2, PASSES[n] - 1, 0), (1, PASSES[n] - 1, 1)]
-    if n in (1024, 2048):
+    if n in (1024, 2048, 4096):
         variants.append((1, PASSES[n], 0))  # the swap schedule
         variants.append((2, PASSES[n], 0))  # swap schedule, two A-scans per lane
     for v in variants:
-        if not small and v[0] == 1 and v[1] != PASSES[n] - 1:
+        if not small and v[0] == 1 and v[1] < PASSES[n] - 1:
             continue
         g.ctx.set_launch_variant(*v)
         e = db_error(g.frame(raw), ref)

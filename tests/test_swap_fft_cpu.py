@@ -194,3 +194,121 @@ def test_planted_address_constant_conflicts():
     _, acc = run(2048, np.ones(2048, complex), skew=True)
     worst = max(conflict_cycles(a[64 * w:64 * w + 64]) for a in acc["E1 read"] for w in range(4))
     assert worst > 2
+
+
+# ------------------------------------------------------------------ N = 4096
+#
+# Radices 8, 8, 8, 8 with n = 512 d0 + 64 d1 + 8 d2 + r2 and
+# k = k0 + 8 k1 + 64 k2 + 512 k3 (chain_kernels.hpp, Swap4096 / fft_swap4096):
+# exchange 1 through LDS at (t & 63) + 64 k0 + 512 (t >> 6), read at
+# t + 512 d1; exchange 2 inside each wavefront, register bits 2, 1, 0 with
+# lane bits 5, 4, 3 (lane-half swaps and a row rotation by 8 with a select);
+# exchange 3 through LDS at k0 + 8 k1 + 64 k2 + 513 r2, read at t + 513 r2.
+
+PADN4096 = 4381  # Geo<4096>::PADN
+
+
+def row_ror8(a):
+    """DPP row_ror:8 on 64 lanes: lane i of each 16-lane row reads lane
+    (i + 8) mod 16 of that row."""
+    out = np.empty_like(a)
+    for r in range(4):
+        row = a[16 * r:16 * r + 16]
+        out[16 * r:16 * r + 16] = np.roll(row, -8)
+    return out
+
+
+def swap_bit3(a, b, lanes, plant=False):
+    """Register bit 0 (a, b) trades places with lane bit 3: a' = hi ? ror8(b)
+    : a, b' = hi ? b : ror8(a), hi = lane bit 3."""
+    hi = (lanes & 8) != 0
+    if plant:
+        hi = ~hi
+    pa, pb = row_ror8(a), row_ror8(b)
+    return np.where(hi, pb, a), np.where(hi, b, pa)
+
+
+def run4096(x, e3=513, plant=None):
+    n, tpa = 4096, 512
+    w = lambda p: np.exp(2j * np.pi * p / n)  # noqa: E731
+    t = np.arange(tpa)
+    v = np.array([[x[i + tpa * e] for e in range(8)] for i in range(tpa)], complex)
+    acc = {"E1 write": [], "E1 read": [], "E3 write": [], "E3 read": []}
+    # pass 0: DFT-8 over d0, twiddle w_N^(r0 k0), r0 = t
+    v = np.array([idft(row) for row in v])
+    v[:, 1:] *= np.array([[w(i * e) for e in range(1, 8)] for i in t])
+    lds = np.full(PADN4096, np.nan + 0j)
+    for e in range(8):
+        addr = (t & 63) + 64 * e + 512 * (t >> 6)
+        acc["E1 write"].append(addr)
+        lds[addr] = v[:, e]
+    for e in range(8):
+        addr = t + 512 * e
+        acc["E1 read"].append(addr)
+        v[:, e] = lds[addr]
+    # pass 1: DFT-8 over d1, twiddle w_512^(r1 k1), r1 = t & 63
+    v = np.array([idft(row) for row in v])
+    r1 = (t & 63) if plant != "r1" else (t & 7)
+    v[:, 1:] *= np.array([[w(8 * r1[i] * e) for e in range(1, 8)] for i in t])
+    # exchange 2 in each wavefront
+    for wave in range(tpa // 64):
+        ln = slice(64 * wave, 64 * wave + 64)
+        lanes = np.arange(64)
+        for e in range(4):
+            v[ln, e], v[ln, e + 4] = permlane32(v[ln, e], v[ln, e + 4])
+        for e in (0, 1, 4, 5):
+            v[ln, e], v[ln, e + 2] = permlane16(v[ln, e], v[ln, e + 2])
+        for e in (0, 2, 4, 6):
+            v[ln, e], v[ln, e + 1] = swap_bit3(v[ln, e], v[ln, e + 1], lanes, plant == "bit3")
+    # pass 2: DFT-8 over d2, twiddle w_64^(r2 k2), r2 = t & 7
+    v = np.array([idft(row) for row in v])
+    v[:, 1:] *= np.array([[w(64 * (i & 7) * e) for e in range(1, 8)] for i in t])
+    # exchange 3 and pass 3
+    lds = np.full(PADN4096, np.nan + 0j)
+    for e in range(8):
+        addr = (t >> 6) + 8 * ((t >> 3) & 7) + 64 * e + e3 * (t & 7)
+        acc["E3 write"].append(addr)
+        lds[addr] = v[:, e]
+    for e in range(8):
+        addr = t + e3 * e
+        acc["E3 read"].append(addr)
+        v[:, e] = lds[addr]
+    v = np.array([idft(row) for row in v])
+    X = np.zeros(n, complex)
+    for e in range(8):
+        X[t + tpa * e] = v[:, e]
+    return X, acc
+
+
+def test_swap_schedule_4096_is_the_inverse_dft():
+    rng = np.random.default_rng(4096)
+    x = rng.normal(size=4096) + 1j * rng.normal(size=4096)
+    X, _ = run4096(x)
+    ref = np.fft.ifft(x) * 4096
+    assert np.max(np.abs(X - ref)) < 1e-12 * np.abs(ref).max()
+
+
+def test_swap_schedule_4096_exchanges_one_to_one_in_bounds_conflict_free():
+    _, acc = run4096(np.ones(4096, complex))
+    for name, per_e in acc.items():
+        allad = np.concatenate(per_e)
+        assert allad.min() >= 0 and allad.max() < PADN4096, name
+        assert len(np.unique(allad)) == 4096, name
+        for addr in per_e:
+            for wave in range(8):
+                assert conflict_cycles(addr[64 * wave:64 * wave + 64]) == 2, name
+
+
+@pytest.mark.parametrize("plant", ["bit3", "r1"])
+def test_planted_4096_mistakes_change_the_result(plant):
+    rng = np.random.default_rng(3)
+    x = rng.normal(size=4096) + 1j * rng.normal(size=4096)
+    X, _ = run4096(x, plant=plant)
+    ref = np.fft.ifft(x) * 4096
+    assert not np.max(np.abs(X - ref)) < 1e-6 * np.abs(ref).max()
+
+
+def test_planted_4096_even_exchange3_stride_conflicts():
+    _, acc = run4096(np.ones(4096, complex), e3=512)
+    worst = max(conflict_cycles(a[64 * w:64 * w + 64]) for a in acc["E3 write"] for w in range(8))
+    assert worst > 2

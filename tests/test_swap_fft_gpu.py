@@ -1,7 +1,8 @@
 """The swap schedule of the chain's FFT (launch variant "small-radix pass =
-passes", N = 1024 and 2048): radices 8, 8, S, 8 with the exchange into the
-small-radix pass made by v_permlane32_swap / v_permlane16_swap inside each
-wavefront instead of through LDS.  Every case runs the swap variant and the
+passes", N = 1024, 2048 and 4096): radices 8, 8, S, 8 (8, 8, 8, 8 at 4096)
+with the second exchange made inside each wavefront instead of through LDS
+(v_permlane32_swap / v_permlane16_swap, and at 4096 a DPP row rotation by 8
+with a select for the third register bit).  Every case runs the swap variant and the
 default Stockham variant and checks both against the FP64 oracle, and the
 two against each other (same arithmetic up to the FFT's rounding order).
 """
@@ -18,7 +19,7 @@ from tests.helpers import (DB_TOL, LIN_TOL, GpuChain, db_error, db_lin_error, db
 
 pytestmark = pytest.mark.gpu
 
-PASSES = 4  # N = 1024 and 2048
+PASSES = 4  # N = 1024, 2048 and 4096
 METHODS = {"none": None, "linear": O.LINEAR, "cubic": O.CUBIC, "lanczos": O.LANCZOS}
 
 
@@ -46,7 +47,7 @@ def _both(p, raw, t, setup=None, wg_per_cu=0, ascans_per_lane=1):
     return out
 
 
-@pytest.mark.parametrize("n", [1024, 2048])
+@pytest.mark.parametrize("n", [1024, 2048, 4096])
 @pytest.mark.parametrize("method", ["none", "linear", "cubic", "lanczos"])
 def test_swap_log_output(n, method):
     a = 67  # odd: the last workgroup iteration is partly empty
@@ -64,7 +65,7 @@ def test_swap_log_output(n, method):
     assert e_d <= DB_TOL
 
 
-@pytest.mark.parametrize("n", [1024, 2048])
+@pytest.mark.parametrize("n", [1024, 2048, 4096])
 @pytest.mark.parametrize("dtype", [np.uint8, np.uint16, np.uint32, np.float32])
 def test_swap_linear_output_and_input_types(n, dtype):
     a = 40
@@ -81,7 +82,7 @@ def test_swap_linear_output_and_input_types(n, dtype):
     assert e_s <= LIN_TOL and e_d <= LIN_TOL
 
 
-@pytest.mark.parametrize("n", [1024, 2048])
+@pytest.mark.parametrize("n", [1024, 2048, 4096])
 @pytest.mark.parametrize("window", [0, 1, 64, 300])
 def test_swap_with_dc_removal(n, window):
     a = 33
@@ -141,7 +142,7 @@ def test_swap_every_occupancy(wg_per_cu):
     assert db_error(swap, ref) <= DB_TOL and db_lin_error(swap, ref) <= LIN_TOL
 
 
-@pytest.mark.parametrize("n", [1024, 2048])
+@pytest.mark.parametrize("n", [1024, 2048, 4096])
 @pytest.mark.parametrize("case", ["dispersion", "pairs", "pairs-lanczos", "dc"])
 def test_swap_two_ascans_per_lane(n, case):
     """Two A-scans per lane: with dispersion each takes its own FFT (the
@@ -165,7 +166,7 @@ def test_swap_two_ascans_per_lane(n, case):
 
 def test_swap_not_built_elsewhere():
     from octproengine_amd import _capi
-    for n, passes in ((256, 3), (512, 3), (4096, 4)):
+    for n, passes in ((256, 3), (512, 3)):
         p = _params(n, 16, "cubic")
         t = standard_tables(n)
         g = GpuChain(p, t.curve, t.window, t.phasor)
