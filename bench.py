@@ -138,9 +138,10 @@ def ragged_workload(seed=0, count=512):
 
 
 def ragged_measurement(capi, S, torch, stream, reps=3):
-    """One persistent launch over the mixed-length batch; A-scans/s of the
-    whole batch and the p50/p99 per-request latency inside the launch, plus
-    the same requests as one ope_hip_process_device launch each."""
+    """The mixed-length batch, one persistent launch per signal length over
+    a row map of its requests; A-scans/s of the whole batch and the p50/p99
+    per-request latency from the start of the first launch, plus the same
+    requests as one ope_hip_process_device launch each."""
     import numpy as np
     reqs = ragged_workload()
     ctxs = {}

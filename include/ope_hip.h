@@ -331,8 +331,10 @@ OPE_HIP_API int ope_hip_op_fpn_removal(ope_hip_ctx* ctx, const float* input, con
 /* ---- Ragged batches.  No counterpart in the reference, whose backends
  * process one fixed-size frame per call (ProcessingBackend::process,
  * backend_interface.h:27).  A ragged batch runs many requests of different
- * signal lengths and A-scan counts in one persistent launch: for serving
- * callers that queue requests from several scan protocols. */
+ * signal lengths and A-scan counts in one call: one persistent launch per
+ * signal length present, each that length's own chain kernel (its tuned
+ * launch variant and occupancy) over a row map of the requests' A-scans,
+ * for serving callers that queue requests from several scan protocols. */
 
 /* One request: n_ascans A-scans of signal_length samples from d_in (device,
  * [n_ascans][signal_length] in the contexts' input type, 16-byte aligned)
@@ -353,20 +355,21 @@ typedef struct ope_hip_ragged ope_hip_ragged;
  * interpolation method, DC removal and input sample width, and live on one
  * device.  A context with an FPN determination or background recording
  * pending is rejected (OPE_HIP_ESTATE): run it through process_device
- * first.  chunk_ascans: A-scans per task (<= 0: 2^18 samples per task).
- * Errors are reported through ope_hip_last_error(NULL). */
+ * first.  chunk_ascans: unused, must be >= 0 (the launches stride over
+ * single A-scans).  Errors are reported through ope_hip_last_error(NULL). */
 OPE_HIP_API int ope_hip_ragged_create(ope_hip_ctx* const* ctxs, int n_ctx, const ope_hip_request* reqs,
                                       int n_reqs, int chunk_ascans, ope_hip_ragged** out);
 
-/* One launch over the whole batch on `stream` (NULL: the HIP null stream).
+/* The whole batch on `stream` (NULL: the HIP null stream): one launch per
+ * signal length, in the order of each length's first request.
  * Asynchronous, allocates nothing and may be captured in a graph.  Runs of
- * one batch must not overlap (they share its task counters). */
+ * one batch must not overlap (they share its completion counters). */
 OPE_HIP_API int ope_hip_ragged_run(ope_hip_ragged* batch, void* stream);
 
 /* Per-request latency of the last completed run: microseconds from the
- * moment the first task was taken to the completion of the request's last
- * A-scan (0 for empty requests).  Call after the run has finished.
- * latency_us has n_reqs entries. */
+ * start of the run's first launch to the moment the request's last A-scan
+ * had its outputs issued (0 for empty requests).  Call after the run has
+ * finished.  latency_us has n_reqs entries. */
 OPE_HIP_API int ope_hip_ragged_latencies(ope_hip_ragged* batch, double* latency_us);
 
 OPE_HIP_API const char* ope_hip_ragged_last_error(const ope_hip_ragged* batch);

@@ -37,7 +37,6 @@ HIP_FLAGS = [
     "-Wno-pass-failed",
 ]
 CHAIN_LENGTHS = (256, 512, 1024, 2048, 4096)
-RAGGED_TAPS = (1, 2, 4, 8, 16)
 
 LIB_HIP = PKG / "libope_hip.so"
 LIB_CORE = PKG / "libope.so"
@@ -68,9 +67,9 @@ def _run_parallel(cmds: list[list[str]], jobs: int) -> None:
 
 def build_hip(verbose_resources: bool = False) -> Path:
     """libope_hip.so: the C ABI host code, one object per signal length
-    holding that length's chain-kernel instantiations and one object per tap
-    count holding the ragged-batch kernels, one object per signal length
-    holding the autofocus kernels, compiled in parallel."""
+    and A-scans per lane holding that length's chain-kernel instantiations,
+    one object per signal length holding the autofocus kernels, compiled in
+    parallel."""
     hip_dir = CSRC / "hip"
     OBJ_DIR.mkdir(exist_ok=True)
     extra = ["-Rpass-analysis=kernel-resource-usage"] if verbose_resources else []
@@ -84,11 +83,6 @@ def build_hip(verbose_resources: bool = False) -> Path:
             cmds.append([HIPCC, *HIP_FLAGS, *extra, f"-I{INCLUDE}", f"-DOPE_CHAIN_N={n}", f"-DOPE_CHAIN_AS={a_s}",
                          "-c", str(hip_dir / "chain_instances.hip"), "-o", str(obj)])
             objs.append(obj)
-    for t in RAGGED_TAPS:
-        obj = OBJ_DIR / f"ragged_t{t}.o"
-        cmds.append([HIPCC, *HIP_FLAGS, *extra, f"-I{INCLUDE}", f"-DOPE_RAGGED_T={t}", "-c",
-                     str(hip_dir / "ragged_instances.hip"), "-o", str(obj)])
-        objs.append(obj)
     for n in CHAIN_LENGTHS:
         obj = OBJ_DIR / f"autofocus_n{n}.o"
         cmds.append([HIPCC, *HIP_FLAGS, *extra, f"-I{INCLUDE}", f"-DOPE_AF_N={n}", "-c",

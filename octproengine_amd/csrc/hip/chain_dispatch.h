@@ -4,7 +4,6 @@
 #pragma once
 
 #include "chain_kernels.hpp"
-#include "ragged_kernels.hpp"
 #include "autofocus_kernels.hpp"
 
 namespace ope_hip {
@@ -25,14 +24,6 @@ struct ChainLaunch {
 // and AS = 1, 2 (chain_instances.hip).
 template <int N, int AS>
 ChainLaunch pick_chain(int taps, int mode, int iw, int sched);
-
-using RaggedFn = void (*)(RaggedArgs);
-
-// Ragged-batch kernel for taps T, mode (0 or M_DC) and input width iw.
-// Explicitly instantiated for T = 1, 2, 4, 8, 16 (ragged_instances.hip
-// compiled with -DOPE_RAGGED_T=T).
-template <int T>
-RaggedFn pick_ragged_t(int mode, int iw);
 
 using AutofocusFn = void (*)(AutofocusArgs);
 

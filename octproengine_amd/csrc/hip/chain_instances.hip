@@ -4,7 +4,10 @@
 //          bytes x every radix placement, plus the FPN-statistics modes with
 //          the default placement;
 //  AS = 2: taps x modes {plain, DC, pairs} x widths, default placement;
-//  both:   for N = 1024 and 2048 also the swap schedule (sched = passes).
+//  both:   for N = 1024 and 2048 also the swap schedule (sched = passes);
+//  row maps (ragged batches): the default launch only, i.e. modes {plain,
+//          DC} at AS = 1 and pairs at AS = 2, with the swap schedule where
+//          it is built and the last-pass placement elsewhere.
 #include "chain_dispatch.h"
 
 #if !defined(OPE_CHAIN_N) || !defined(OPE_CHAIN_AS)
@@ -80,9 +83,22 @@ ChainLaunch pick_chain(int taps, int mode, int iw, int sched) {
                 }
             }
             break;
-        default:
+        case M_DC | M_STATS:
             if constexpr (AS == 1) l.fn = pick_width<N, M_DC | M_STATS, 1, LAST>(taps, iw);
             break;
+        default: {
+            // row maps: the default schedule only (sched < 0 where it is
+            // the last-pass placement)
+            constexpr int DEF = (N == 1024 || N == 2048) ? LAST + 1 : LAST;
+            if (sched != DEF && !(sched < 0 && DEF == LAST)) break;
+            if constexpr (AS == 1) {
+                if (mode == M_ROWS) l.fn = pick_width<N, M_ROWS, 1, DEF>(taps, iw);
+                if (mode == (M_ROWS | M_DC)) l.fn = pick_width<N, M_ROWS | M_DC, 1, DEF>(taps, iw);
+            } else {
+                if (mode == (M_ROWS | M_PAIR)) l.fn = pick_width<N, M_ROWS | M_PAIR, 2, DEF>(taps, iw);
+            }
+            break;
+        }
     }
     return l;
 }

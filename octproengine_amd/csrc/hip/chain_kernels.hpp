@@ -162,6 +162,14 @@ enum : int {
                      // holds fpn - offset * IFFT(S * phasor) (see ChainArgs)
 };
 
+// One A-scan of a ragged batch (M_ROWS): where it is read and written, and
+// the request it belongs to.
+struct RowRef {
+    const void* in;          // N samples
+    float* out;              // N/2 outputs
+    long long req;
+};
+
 struct ChainArgs {
     const void* in;          // raw samples [n_ascans][N]
     float* out;              // [n_ascans][N/2]
@@ -193,6 +201,16 @@ struct ChainArgs {
     long long row0;          // accumulated row index of A-scan 0 of this call
     int seg_w;               // rows per segment
     int stat_rows;           // A-scans of this call that enter the statistics
+    // Row map (M_ROWS kernels): A-scan a of the call reads
+    // rows[a].in and writes rows[a].out instead of rows a of in and out.
+    // Once an A-scan's outputs are issued, its first lane raises
+    // finish[req] to the wall clock (an atomic max without return, so the
+    // wavefront does not wait on it; the clock only grows, so no reset is
+    // needed between runs).  start, where not null, is stamped by workgroup
+    // 0 when it begins.
+    const RowRef* rows;
+    unsigned long long* finish;
+    unsigned long long* start;
 };
 
 // input classes (numeric values match ope::DataType)
@@ -804,6 +822,8 @@ enum : int {
     M_STATS = 2,   // FPN determination: accumulate segment statistics, no output
     M_PAIR = 4,    // no dispersion: the lane's two A-scans as the real and
                    // imaginary part of one complex FFT (AS = 2)
+    M_ROWS = 8,    // A-scans through the row map of ChainArgs (ragged
+                   // batches), not rows of one frame
 };
 
 // Rolling-average DC removal of one A-scan (cpu_kernels.tpp:56-82), staged
@@ -1219,13 +1239,33 @@ __device__ __forceinline__ void chain_body(const ChainArgs A, float2* bufA, floa
         for (int o = 0; o < 4; ++o) sx[u][o] = sy[u][o] = sxx[u][o] = 0.0;
     }
 
+    // Row map (ragged batches): nin[u] is the input row of the lane's next
+    // A-scan to be loaded, read one group ahead of the sample loads that
+    // use it.  Its loads are unconditional (indices past the range clamp
+    // to its last A-scan, which is then read again and not used): with the
+    // same number of loads in flight on every path, the epilogue waits only
+    // for its own row entry, not for the prefetches issued after it.
+    constexpr bool rowmap = (MODE & M_ROWS) != 0;
+    static_assert(!rowmap || !STATS, "row maps are for the throughput mode");
+    auto clamp_row = [&](long long a) { return a < a_end ? a : a_end - 1; };
+    const void* nin[AS];
     uint32_t raw[AS][8];
+    auto load_next = [&](int u, long long an) {
+        if constexpr (rowmap) {
+            load_raw<IW>(nin[u], N, 0, t, raw[u]);
+            nin[u] = gmem(A.rows)[clamp_row(an + a_step)].in;
+        } else {
+            if (an < a_end) load_raw<IW>(A.in, N, an, t, raw[u]);
+        }
+    };
 #pragma unroll
     for (int u = 0; u < AS; ++u) {
 #pragma unroll
         for (int i = 0; i < 8; ++i) raw[u][i] = 0;
         const long long a = a_begin + slot * AS + u;
-        if (a < a_end) load_raw<IW>(A.in, N, a, t, raw[u]);
+        nin[u] = nullptr;
+        if constexpr (rowmap) nin[u] = gmem(A.rows)[clamp_row(a)].in;
+        load_next(u, a);
     }
 
     // Staging of A-scan group a: convert the raw words (loaded one group
@@ -1275,8 +1315,7 @@ __device__ __forceinline__ void chain_body(const ChainArgs A, float2* bufA, floa
                 dst[0] = make_float4(x8[0], x8[1], x8[2], x8[3]);
                 dst[1] = make_float4(x8[4], x8[5], x8[6], x8[7]);
             }
-            const long long an = a + a_step + slot * AS + u;
-            if (an < a_end) load_raw<IW>(A.in, N, an, t, raw[u]);
+            load_next(u, a + a_step + slot * AS + u);
         }
     };
     if constexpr (PIPE) {
@@ -1288,6 +1327,17 @@ __device__ __forceinline__ void chain_body(const ChainArgs A, float2* bufA, floa
         // the staging shares buffer B, which the previous A-scan's last FFT
         // pass may still be reading in other wavefronts
         if constexpr (L::SWAP_TOP_BARRIER) __syncthreads();
+        // row map: this group's output rows and requests, read here so that
+        // the epilogue's stores do not wait on them
+        RowRef rr[AS];
+#pragma unroll
+        for (int u = 0; u < AS; ++u) {
+            rr[u] = RowRef{nullptr, nullptr, 0};
+            if constexpr (rowmap) {
+                const OPE_GLOBAL RowRef* e = gmem(A.rows) + clamp_row(a0 + slot * AS + u);
+                rr[u] = RowRef{nullptr, e->out, e->req};
+            }
+        }
         // ---- stage converted samples in LDS
         float xv[AS][8];
         if constexpr (!PIPE) {
@@ -1300,8 +1350,7 @@ __device__ __forceinline__ void chain_body(const ChainArgs A, float2* bufA, floa
                     dst[1] = make_float4(xv[u][4], xv[u][5], xv[u][6], xv[u][7]);
                 }
                 // prefetch the next A-scan while this one is processed
-                const long long an = a0 + a_step + slot * AS + u;
-                if (an < a_end) load_raw<IW>(A.in, N, an, t, raw[u]);
+                load_next(u, a0 + a_step + slot * AS + u);
             }
         }
 
@@ -1341,9 +1390,11 @@ __device__ __forceinline__ void chain_body(const ChainArgs A, float2* bufA, floa
         if constexpr (PRIO) __builtin_amdgcn_s_setprio(0);
 
         // ---- inverse FFT: NP Stockham passes
+        // (a row map stages past the range too, the clamped last A-scan
+        // again, so that its loads are unconditional)
         auto next = [&]() {
             if constexpr (PIPE) {
-                if (a0 + a_step < a_end) {
+                if (rowmap || a0 + a_step < a_end) {
                     if constexpr (SPLIT)
                         stage_prefix();
                     else
@@ -1457,24 +1508,33 @@ __device__ __forceinline__ void chain_body(const ChainArgs A, float2* bufA, floa
                     for (int o = 0; o < 4; ++o) res[o] = fmaf(ep_a, __fsqrt_rn(m2[o]), ep_b);
                 }
                 if (active) {
-                    long long row = a;
-                    if (flags & F_FLIP) {
-                        // cuda_kernels.cu:553-571: B-scans 0, 2, 4, ... reversed
-                        const long long b = a / A.ascans_per_bscan;
-                        if ((b & 1) == 0 && b + 1 < A.flip_bscans)
-                            row = b * A.ascans_per_bscan + (A.ascans_per_bscan - 1 - (a - b * A.ascans_per_bscan));
+                    OPE_GLOBAL float* dst;
+                    if (rowmap) {
+                        dst = gmem(rr[u].out) + t;
+                    } else {
+                        long long row = a;
+                        if (flags & F_FLIP) {
+                            // cuda_kernels.cu:553-571: B-scans 0, 2, 4, ... reversed
+                            const long long b = a / A.ascans_per_bscan;
+                            if ((b & 1) == 0 && b + 1 < A.flip_bscans)
+                                row = b * A.ascans_per_bscan + (A.ascans_per_bscan - 1 - (a - b * A.ascans_per_bscan));
+                        }
+                        dst = gmem(A.out) + row * G::HALF + t;
                     }
-                    OPE_GLOBAL float* dst = gmem(A.out) + row * G::HALF + t;
 #pragma unroll
                     for (int o = 0; o < 4; ++o) {
                         // bgv is 0 without background subtraction: x - 0 == x exactly
                         __builtin_nontemporal_store(res[o] - bgv[o], dst + o * TPA);
                     }
+                    // row map: the request's completion stamp, once the
+                    // stores are issued (not waited for: the stamp is a
+                    // latency figure, not a completion signal)
+                    if (rowmap && t == 0) atomicMax(A.finish + rr[u].req, (unsigned long long)wall_clock64());
                 }
             }
         }
         if constexpr (PIPE && SPLIT) {
-            if (a0 + a_step < a_end) stage_rest(a0 + a_step);
+            if (rowmap || a0 + a_step < a_end) stage_rest(a0 + a_step);
         }
     }
 
@@ -1537,6 +1597,9 @@ chain_kernel(ChainArgs A) {
         a_begin = (long long)blockIdx.x * SLOTS;
         a_end = A.n_ascans;
         a_step = (long long)gridDim.x * SLOTS;
+        if constexpr ((MODE & M_ROWS) != 0) {
+            if (A.start && blockIdx.x == 0 && threadIdx.x == 0) *A.start = wall_clock64();
+        }
     }
     chain_body<N, T, MODE, IW, AS, WG, SCHED>(A, bufA, bufB, xsep, scan_part, mm_part, a_begin, a_end, a_step);
 }

@@ -158,23 +158,23 @@ struct ope_hip_ctx {
     std::atomic<bool> foreign_work{false};
 };
 
-// A ragged batch: snapshot of the class arguments, the task list and the
-// self-resetting counters of ragged_kernel.
+// A ragged batch: per signal length present, the chain arguments of its
+// context (snapshot) with the row map of that length's A-scans; the
+// per-request completion stamps.
 struct ope_hip_ragged {
     int device = 0;
     std::vector<ope_hip_ctx*> ctxs;  // whose tables the batch reads
     std::string err;
     int n_reqs = 0;
-    int n_tasks = 0;
-    int grid = 0;
-    RaggedFn fn = nullptr;
     double tick_hz = 0.0;
-    std::vector<int> req_tasks;  // tasks per request
-    ChainArgs* d_cls = nullptr;
-    RaggedRequest* d_reqs = nullptr;
-    RaggedTask* d_tasks = nullptr;
-    unsigned* d_counters = nullptr;
-    unsigned* d_remaining = nullptr;
+    std::vector<long long> req_ascans;
+    struct Class {
+        ope_hip_ctx* ctx;
+        ChainArgs args;      // the row map of this length's A-scans
+        long long ascans;
+    };
+    std::vector<Class> classes;  // in the order of their first request
+    RowRef* d_rows = nullptr;
     unsigned long long* d_finish = nullptr;
     unsigned long long* d_start = nullptr;
 };
@@ -578,6 +578,10 @@ int launch_variant(ope_hip_ctx* c, const ChainArgs& a, int mode, int as, int sch
     return OPE_HIP_OK;
 }
 
+// The untuned launch: pairs where pairable; the swap schedule where it is
+// built (N = 1024, 2048: faster in every configuration measured).
+int default_sched(int n) { return (n == 1024 || n == 2048) ? fft_passes(n) : -1; }
+
 int launch_chain(ope_hip_ctx* c, const ChainArgs& a, bool stats, long long work_ascans, hipStream_t s) {
     const int mode = (c->p.dc_removal_enabled ? M_DC : 0) | (stats ? M_STATS : 0);
     const int iw = (int)input_bytes(c->p.input_type);
@@ -585,10 +589,7 @@ int launch_chain(ope_hip_ctx* c, const ChainArgs& a, bool stats, long long work_
         for (const auto& t : c->tuning)
             if (t.taps == c->taps && t.mode == mode && t.iw == iw)
                 return launch_variant(c, a, mode, t.as, t.sched, t.per_cu, work_ascans, s);
-    // untuned: pairs where pairable; the swap schedule where it is built
-    // (N = 1024, 2048: faster in every configuration measured)
-    const int sched = (c->n == 1024 || c->n == 2048) ? fft_passes(c->n) : -1;
-    return launch_variant(c, a, mode, pairable(a, mode) ? 2 : 1, sched, 0, work_ascans, s);
+    return launch_variant(c, a, mode, pairable(a, mode) ? 2 : 1, default_sched(c->n), 0, work_ascans, s);
 }
 
 int zero_acc(ope_hip_ctx* c, hipStream_t s) {
@@ -850,8 +851,7 @@ int autofocus_pass(ope_hip_ctx* c, const void* d_in, long long n_ascans, const d
 void free_ragged(ope_hip_ragged* r) {
     if (!r) return;
     (void)hipSetDevice(r->device);
-    for (void* q : {(void*)r->d_cls, (void*)r->d_reqs, (void*)r->d_tasks, (void*)r->d_counters,
-                    (void*)r->d_remaining, (void*)r->d_finish, (void*)r->d_start})
+    for (void* q : {(void*)r->d_rows, (void*)r->d_finish, (void*)r->d_start})
         if (q) (void)hipFree(q);
     delete r;
 }
@@ -877,7 +877,7 @@ ChainArgs steady_args(ope_hip_ctx* c) {
 }
 
 int build_ragged(ope_hip_ragged* r, ope_hip_ctx* const* ctxs, int n_ctx, const ope_hip_request* reqs,
-                 int n_reqs, int chunk_ascans) {
+                 int n_reqs) {
     if (!ctxs || n_ctx < 1 || n_ctx > 5) return ragged_fail(r, OPE_HIP_EINVAL, "n_ctx must be in 1..5");
     if (n_reqs < 0 || (n_reqs > 0 && !reqs)) return ragged_fail(r, OPE_HIP_EINVAL, "bad request array");
     for (int i = 0; i < n_ctx; ++i) {
@@ -905,10 +905,11 @@ int build_ragged(ope_hip_ragged* r, ope_hip_ctx* const* ctxs, int n_ctx, const o
     r->device = c0->device;
     r->ctxs.assign(ctxs, ctxs + n_ctx);
     r->n_reqs = n_reqs;
-    const size_t iw = input_bytes(c0->p.input_type);
-    std::vector<RaggedRequest> hreq(n_reqs);
-    std::vector<RaggedTask> tasks;
-    r->req_tasks.assign(n_reqs, 0);
+    r->req_ascans.assign(n_reqs, 0);
+    // requests by length class, in request order; classes in the order of
+    // their first request
+    std::vector<int> req_cls(n_reqs, -1), order;
+    std::vector<long long> cls_ascans(n_ctx, 0);
     for (int q = 0; q < n_reqs; ++q) {
         const ope_hip_request& rq = reqs[q];
         int cls = -1;
@@ -920,23 +921,30 @@ int build_ragged(ope_hip_ragged* r, ope_hip_ctx* const* ctxs, int n_ctx, const o
             return ragged_fail(r, OPE_HIP_EINVAL, "request %d: null device pointer", q);
         if (reinterpret_cast<uintptr_t>(rq.d_in) % 16 || reinterpret_cast<uintptr_t>(rq.d_out) % 4)
             return ragged_fail(r, OPE_HIP_EINVAL, "request %d: d_in must be 16-byte and d_out 4-byte aligned", q);
-        hreq[q] = RaggedRequest{rq.d_in, rq.d_out, rq.n_ascans};
-        const int n = rq.signal_length;
-        const long long chunk = chunk_ascans > 0 ? chunk_ascans : std::max(1, (1 << 18) / n);
-        const long long nt = (rq.n_ascans + chunk - 1) / chunk;
-        if ((long long)tasks.size() + nt > (1LL << 30))
-            return ragged_fail(r, OPE_HIP_EINVAL, "too many tasks; raise chunk_ascans");
-        for (long long k = 0; k < nt; ++k) {
-            const long long a0 = k * chunk, a1 = std::min<long long>(rq.n_ascans, a0 + chunk);
-            tasks.push_back(RaggedTask{q, cls, n, (int)nt, a0, a1});
-        }
-        r->req_tasks[q] = (int)nt;
+        req_cls[q] = cls;
+        r->req_ascans[q] = rq.n_ascans;
+        if (rq.n_ascans > 0 && cls_ascans[cls] == 0) order.push_back(cls);
+        cls_ascans[cls] += rq.n_ascans;
     }
-    r->n_tasks = (int)tasks.size();
-    std::vector<ChainArgs> cls(n_ctx);
-    for (int i = 0; i < n_ctx; ++i) cls[i] = steady_args(ctxs[i]);
-    std::vector<unsigned> remaining(std::max(n_reqs, 1), 0u);
-    for (int q = 0; q < n_reqs; ++q) remaining[q] = (unsigned)r->req_tasks[q];
+    // the row map: each class's A-scans contiguous, requests in order
+    long long total = 0;
+    std::vector<long long> cls_first(n_ctx, 0);
+    for (int cls : order) {
+        cls_first[cls] = total;
+        total += cls_ascans[cls];
+    }
+    std::vector<RowRef> rows((size_t)total);
+    std::vector<long long> fill(cls_first);
+    for (int q = 0; q < n_reqs; ++q) {
+        const ope_hip_request& rq = reqs[q];
+        const size_t iw = input_bytes(ctxs[req_cls[q]]->p.input_type);
+        for (long long a = 0; a < rq.n_ascans; ++a) {
+            RowRef& e = rows[(size_t)fill[req_cls[q]]++];
+            e.in = static_cast<const char*>(rq.d_in) + (size_t)a * rq.signal_length * iw;
+            e.out = rq.d_out + (size_t)a * (rq.signal_length / 2);
+            e.req = q;
+        }
+    }
 
 #define RAGGED_TRY(expr)                                                                         \
     do {                                                                                         \
@@ -946,40 +954,25 @@ int build_ragged(ope_hip_ragged* r, ope_hip_ctx* const* ctxs, int n_ctx, const o
                                "%s: %s", #expr, hipGetErrorString(e_));                          \
     } while (0)
     RAGGED_TRY(hipSetDevice(r->device));
-    const size_t nq = (size_t)std::max(n_reqs, 1), ntk = (size_t)std::max(r->n_tasks, 1);
-    RAGGED_TRY(hipMalloc(&r->d_cls, n_ctx * sizeof(ChainArgs)));
-    RAGGED_TRY(hipMalloc(&r->d_reqs, nq * sizeof(RaggedRequest)));
-    RAGGED_TRY(hipMalloc(&r->d_tasks, ntk * sizeof(RaggedTask)));
-    RAGGED_TRY(hipMalloc(&r->d_counters, 2 * sizeof(unsigned)));
-    RAGGED_TRY(hipMalloc(&r->d_remaining, nq * sizeof(unsigned)));
+    const size_t nq = (size_t)std::max(n_reqs, 1);
+    RAGGED_TRY(hipMalloc(&r->d_rows, std::max<size_t>(rows.size(), 1) * sizeof(RowRef)));
     RAGGED_TRY(hipMalloc(&r->d_finish, nq * sizeof(unsigned long long)));
     RAGGED_TRY(hipMalloc(&r->d_start, sizeof(unsigned long long)));
-    RAGGED_TRY(hipMemcpy(r->d_cls, cls.data(), n_ctx * sizeof(ChainArgs), hipMemcpyHostToDevice));
-    if (n_reqs > 0)
-        RAGGED_TRY(hipMemcpy(r->d_reqs, hreq.data(), n_reqs * sizeof(RaggedRequest), hipMemcpyHostToDevice));
-    if (r->n_tasks > 0)
-        RAGGED_TRY(hipMemcpy(r->d_tasks, tasks.data(), tasks.size() * sizeof(RaggedTask), hipMemcpyHostToDevice));
-    const unsigned zero2[2] = {0u, 0u};
-    RAGGED_TRY(hipMemcpy(r->d_counters, zero2, sizeof(zero2), hipMemcpyHostToDevice));
-    RAGGED_TRY(hipMemcpy(r->d_remaining, remaining.data(), nq * sizeof(unsigned), hipMemcpyHostToDevice));
+    if (!rows.empty())
+        RAGGED_TRY(hipMemcpy(r->d_rows, rows.data(), rows.size() * sizeof(RowRef), hipMemcpyHostToDevice));
     std::vector<unsigned long long> zt(nq, 0ull);
     RAGGED_TRY(hipMemcpy(r->d_finish, zt.data(), nq * sizeof(unsigned long long), hipMemcpyHostToDevice));
     RAGGED_TRY(hipMemcpy(r->d_start, zt.data(), sizeof(unsigned long long), hipMemcpyHostToDevice));
-
-    RaggedFn fn = nullptr;
-    const int mode = c0->p.dc_removal_enabled ? M_DC : 0;
-    switch (c0->taps) {
-        case 1: fn = pick_ragged_t<1>(mode, (int)iw); break;
-        case 2: fn = pick_ragged_t<2>(mode, (int)iw); break;
-        case 4: fn = pick_ragged_t<4>(mode, (int)iw); break;
-        case 8: fn = pick_ragged_t<8>(mode, (int)iw); break;
-        default: fn = pick_ragged_t<16>(mode, (int)iw); break;
+    for (int cls : order) {
+        ope_hip_ragged::Class k{ctxs[cls], steady_args(ctxs[cls]), cls_ascans[cls]};
+        k.args.in = nullptr;
+        k.args.out = nullptr;
+        k.args.n_ascans = k.ascans;
+        k.args.rows = r->d_rows + cls_first[cls];
+        k.args.finish = r->d_finish;
+        k.args.start = r->classes.empty() ? r->d_start : nullptr;
+        r->classes.push_back(k);
     }
-    r->fn = fn;
-    int per_cu = 0;
-    RAGGED_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(fn), kRaggedWG, 0));
-    if (per_cu < 1) per_cu = 1;
-    r->grid = (int)std::min<long long>((long long)per_cu * c0->cu_count, std::max(r->n_tasks, 1));
     int khz = 0;
     RAGGED_TRY(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, r->device));
     r->tick_hz = (double)khz * 1e3;
@@ -1890,12 +1883,16 @@ int ope_hip_ragged_create(ope_hip_ctx* const* ctxs, int n_ctx, const ope_hip_req
         return OPE_HIP_EINVAL;
     }
     *out = nullptr;
+    if (chunk_ascans < 0) {
+        g_create_err = "chunk_ascans must be >= 0";
+        return OPE_HIP_EINVAL;
+    }
     ope_hip_ragged* r = new (std::nothrow) ope_hip_ragged();
     if (!r) {
         g_create_err = "out of host memory";
         return OPE_HIP_ENOMEM;
     }
-    const int rc = build_ragged(r, ctxs, n_ctx, reqs, n_reqs, chunk_ascans);
+    const int rc = build_ragged(r, ctxs, n_ctx, reqs, n_reqs);
     if (rc) {
         g_create_err = r->err;
         free_ragged(r);
@@ -1907,15 +1904,19 @@ int ope_hip_ragged_create(ope_hip_ctx* const* ctxs, int n_ctx, const ope_hip_req
 
 int ope_hip_ragged_run(ope_hip_ragged* r, void* stream) {
     if (!r) return OPE_HIP_EINVAL;
-    if (r->n_tasks == 0) return OPE_HIP_OK;
+    if (r->classes.empty()) return OPE_HIP_OK;
     hipError_t e = hipSetDevice(r->device);
     if (e != hipSuccess) return ragged_fail(r, OPE_HIP_ERUNTIME, "hipSetDevice: %s", hipGetErrorString(e));
-    for (ope_hip_ctx* c : r->ctxs) c->foreign_work = true;
-    RaggedArgs a{r->d_cls, r->d_reqs, r->d_tasks, r->n_tasks, r->d_counters, r->d_remaining, r->d_finish,
-                 r->d_start};
-    hipLaunchKernelGGL(r->fn, dim3((unsigned)r->grid), dim3(kRaggedWG), 0, static_cast<hipStream_t>(stream), a);
-    e = hipGetLastError();
-    if (e != hipSuccess) return ragged_fail(r, OPE_HIP_ERUNTIME, "ragged launch: %s", hipGetErrorString(e));
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    for (const auto& k : r->classes) {
+        k.ctx->foreign_work = true;
+        // the length's default launch (the row-map kernels are built for
+        // it), as many workgroups as are resident
+        const int mode = M_ROWS | (k.ctx->p.dc_removal_enabled ? M_DC : 0);
+        const int rc = launch_variant(k.ctx, k.args, mode, pairable(k.args, mode) ? 2 : 1, default_sched(k.ctx->n),
+                                      0, k.ascans, s);
+        if (rc) return ragged_fail(r, rc, "length %d: %s", k.ctx->n, k.ctx->err.c_str());
+    }
     return OPE_HIP_OK;
 }
 
@@ -1929,7 +1930,7 @@ int ope_hip_ragged_latencies(ope_hip_ragged* r, double* latency_us) {
     if (e == hipSuccess) e = hipMemcpy(&start, r->d_start, sizeof(start), hipMemcpyDeviceToHost);
     if (e != hipSuccess) return ragged_fail(r, OPE_HIP_ERUNTIME, "latencies: %s", hipGetErrorString(e));
     for (int q = 0; q < r->n_reqs; ++q) {
-        if (r->req_tasks[q] == 0 || fin[q] < start) {
+        if (r->req_ascans[q] == 0 || fin[q] < start) {
             latency_us[q] = 0.0;
             continue;
         }
