@@ -9,9 +9,13 @@ Synthetic seeded fringes (3 reflectors per A-scan) are generated on the GPU.
 
 A step is one frame through ope_hip_process_device (device-resident input
 and output; a ring of 8 distinct frames, 1 GiB of input+output, so the
-256 MB last-level cache cannot hold the working set).  W warm-up steps, then
-K timed steps bracketed by device synchronisation, timed with HIP events on
-the stream the kernels run on.
+256 MB last-level cache cannot hold the working set).  Frames are issued
+round robin over two streams, as the reference's CUDA backend issues its
+frames over its streams, so one frame's launch overlaps the end of the
+previous one.  W warm-up steps, then K timed steps bracketed by device
+synchronisation, timed with HIP events on the main stream, which the second
+stream joins before the closing event.  The same K steps on one stream (one
+launch after the other) are reported beside them.
 
 Printed: one JSON line with the throughput (A-scans/s), the roofline of the
 chain kernel (HBM-bound: algorithmic bytes = N*2 B in + N/2*4 B out per
@@ -251,9 +255,10 @@ def c1_measurement(capi, S, torch, stream, ring=64, reps=20):
     """The 2048 x 512 x 1 uint16 frame (linear resampling, Hann, log, no
     dispersion: two real A-scans per complex FFT), where a frame is only
     4 MiB and launch latency matters: a ring of `ring` such frames processed
-    one launch per frame, as one captured HIP graph of `ring` frames, and
-    as one persistent launch over the whole ring.  The three give identical
-    outputs (same kernel and variant)."""
+    one launch per frame (on one stream, and round robin over two), as one
+    captured HIP graph of `ring` frames, and as one persistent launch over
+    the whole ring.  The four give identical outputs (same kernel and
+    variant)."""
     n, a = 2048, 512
     params = capi.make_params(signal_length=n, ascans_per_bscan=a, bscans_per_buffer=1, input_type=capi.UINT16,
                               resampling_enabled=1, interpolation=capi.LINEAR, windowing_enabled=1,
@@ -287,6 +292,20 @@ def c1_measurement(capi, S, torch, stream, ring=64, reps=20):
 
     us_frames = timed(per_frame, reps) / ring
     ref = out.clone()
+    s2 = torch.cuda.Stream()
+
+    def per_frame_rr():  # frames round robin over two streams
+        fork = torch.cuda.Event()
+        fork.record(stream)
+        s2.wait_event(fork)
+        for f in range(ring):
+            ctx.process_device(ins[f], outs[f], a, sp if f % 2 == 0 else s2.cuda_stream)
+        join = torch.cuda.Event()
+        join.record(s2)
+        stream.wait_event(join)
+
+    us_rr = timed(per_frame_rr, reps) / ring
+    same_rr = bool(torch.equal(out, ref))
     graph = capi.FrameGraph(ctx, ins, outs, a)
     us_graph = timed(lambda: graph.launch(sp), reps) / ring
     same_graph = bool(torch.equal(out, ref))
@@ -294,13 +313,14 @@ def c1_measurement(capi, S, torch, stream, ring=64, reps=20):
     same_one = bool(torch.equal(out, ref))
     graph.close()
     ctx.close()
-    assert same_graph and same_one, "the three forms must give identical outputs"
+    assert same_graph and same_one and same_rr, "the four forms must give identical outputs"
 
     def rate(us):
         return {"value": round(a / (us / 1e6), 1), "unit": "A-scans/s", "us_per_frame": round(us, 2)}
 
     return {"workload": f"2048 x 512 x 1 uint16, linear + Hann + log, ring of {ring} frames",
             "launch_variant": variant, "one_launch_per_frame": rate(us_frames),
+            "one_launch_per_frame_two_streams": rate(us_rr),
             f"graph_{ring}_frames": rate(us_graph), "one_launch_over_ring": rate(us_one)}
 
 
@@ -524,12 +544,18 @@ def main():
     ctx = build_context(cfg, capi, S, hann_window, cos_sin)
     stream = torch.cuda.current_stream()
     sp = stream.cuda_stream
+    # Frames alternate between two streams, as the reference's CUDA backend
+    # issues frames round robin over its streams (cuda_backend.cu:448-449):
+    # the next frame's launch then overlaps the end of the previous one.
+    # The steady-state frames are independent (the FPN profile is fixed).
+    rr_streams = [stream, torch.cuda.Stream()]
 
-    def step(i):
-        ctx.process_device(ins[i % args.ring].data_ptr(), outs[i % args.ring].data_ptr(), frame, sp)
+    def step(i, rr=True):
+        st = rr_streams[i % 2] if rr else stream
+        ctx.process_device(ins[i % args.ring].data_ptr(), outs[i % args.ring].data_ptr(), frame, st.cuda_stream)
 
     for i in range(2):  # FPN determined, then the next determination's statistics
-        step(i)
+        step(i, rr=False)
     torch.cuda.synchronize()
     if args.variant:
         as_, pass_, per_cu = (int(v) for v in args.variant.split(","))
@@ -549,17 +575,33 @@ def main():
     torch.cuda.synchronize()
     ev0 = torch.cuda.Event(enable_timing=True)
     ev1 = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    ev0.record(stream)
-    for i in range(args.steps):
-        step(i)
-    ev1.record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    gpu_ms = ev0.elapsed_time(ev1)
+
+    def timed_steps(rr):
+        """K steps between a synchronize on each side; HIP events on the
+        main stream bracket all of them (the second stream joins it)."""
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ev0.record(stream)
+        if rr:
+            rr_streams[1].wait_event(ev0)
+        for i in range(args.steps):
+            step(i, rr)
+        if rr:
+            join = torch.cuda.Event()
+            join.record(rr_streams[1])
+            stream.wait_event(join)
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, ev0.elapsed_time(ev1)
+
+    wall, gpu_ms = timed_steps(rr=True)
     ms_per_step = gpu_ms / args.steps
     ascans_per_s = frame * args.steps / (gpu_ms / 1e3)
+    # the same steps on one stream: one launch after the other, so the
+    # average per step is the kernel's per-launch duration (its start-up
+    # and drain included), the figure a one-stream kernel trace shows
+    _, gpu_ms_1 = timed_steps(rr=False)
+    ms_per_step_1 = gpu_ms_1 / args.steps
 
     # batched form: the ring's frames in one launch (same kernel, one grid)
     big_in = torch.cat(ins[:4], dim=0)
@@ -627,9 +669,11 @@ def main():
     bytes_per_ascan = n * 2 + (n // 2) * 4
     table_bytes = 24 * n
     algo_bytes = frame * bytes_per_ascan + table_bytes
-    # one chain launch per step in steady state: the timed region's average
-    # per step is the kernel's average launch duration (launch gaps included)
+    # one chain launch per step in steady state: the algorithmic bytes of a
+    # launch over the timed region's time per step (two frames in flight),
+    # and over the one-stream per-launch duration
     achieved = algo_bytes / (ms_per_step / 1e3) / 1e9
+    achieved_1 = algo_bytes / (ms_per_step_1 / 1e3) / 1e9
 
     result = {
         "metric": "A-scans/sec (2048-pt full chain) on 1 MI355X + % HBM roofline vs host CPU",
@@ -648,6 +692,9 @@ def main():
                    "ascans_per_frame": frame, "input": "uint16", "frames_in_ring": args.ring,
                    "wall_s_timed_region": round(wall, 4), "clock_ramp_frames_untimed": ramp_frames,
                    "launch_variant": variant},
+        "frames_in_flight": {"streams": 2, "issue": "round robin, as cuda_backend.cu:448-449"},
+        "one_stream": {"value": round(frame * args.steps / (gpu_ms_1 / 1e3), 1), "unit": "A-scans/s",
+                       "ms_per_step": round(ms_per_step_1, 4), "roofline_frac": round(achieved_1 / HBM_PEAK_GBS, 4)},
         "batched_4_frames_per_launch": {"value": round(batched, 1), "unit": "A-scans/s"},
         "ragged_batch": ragged,
         "volume": volume,
@@ -660,7 +707,8 @@ def main():
         "processor_end_to_end": e2e,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(),
-                     "kernel": f"chain_kernel<2048, 4, 0, 2, {variant['ascans_per_lane']}, {variant['small_radix_pass']}>", "bytes_per_launch": algo_bytes},
+                     "kernel": f"chain_kernel<2048, 4, 0, 2, {variant['ascans_per_lane']}, {variant['small_radix_pass']}>", "bytes_per_launch": algo_bytes,
+                     "us_per_launch": round(ms_per_step * 1e3, 2), "us_per_launch_one_stream": round(ms_per_step_1 * 1e3, 2)},
     }
     if not args.no_cpu_baseline:
         v1, done1, dt1 = cpu_baseline(cfg, args.cpu_seconds, 1)
