@@ -401,14 +401,19 @@ struct FftPlan {
 
 // Twiddles of Stockham passes 1..NPASS-1 for lane t (pass 0 has Ns = 1: all
 // ones); elements with r == 0 need none.  table = exp(+2 pi i k / N).
-template <int N, int SCHED = -1>
+template <int N, int SCHED = -1, int P0 = 1>
 __device__ __forceinline__ void load_twiddles(const float2* table, int t, cf (&tw)[Geo<N>::NPASS][8]) {
     using G = Geo<N>;
     using P = FftPlan<N, SCHED>;
     constexpr int NP = G::NPASS;
     constexpr int TPA = G::TPA;
 #pragma unroll
-    for (int p = 1; p < NP; ++p) {
+    for (int p = 1; p < P0; ++p) {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) tw[p][e] = {1.f, 0.f};  // passes read from an LDS table instead
+    }
+#pragma unroll
+    for (int p = P0; p < NP; ++p) {
         const int R = P::radix(p);
         const int RQ = 8 / R;
         const int ns = P::ns(p);
@@ -422,6 +427,30 @@ __device__ __forceinline__ void load_twiddles(const float2* table, int t, cf (&t
             tw[p][e] = {w.x, w.y};
         }
     }
+}
+
+// N = 4096, one A-scan per lane, Stockham schedule: the twiddles of passes 1
+// and 2 depend on t mod 8 and t mod 64 only, so a workgroup keeps them once
+// in LDS (7 x 8 + 7 x 64 entries, the same floats as the global table) and
+// every lane reads its 7 per pass there, next to the pass's data reads.
+// That frees 28 VGPRs of the lane's loop-invariant tables: with the
+// background constant read in the epilogue and the tap-window starts kept as
+// 16-bit halves, the kernels with up to 2 taps fit 128 VGPRs, i.e. two
+// 512-lane workgroups (four waves per SIMD) per CU instead of one
+// (4096-point frames without resampling 64.1 -> 55-56 us, linear 67.4 -> 57).
+// Cubic taps need 138: squeezed to 128 they spilled 40 bytes per lane and
+// ran no faster (71.8 us against 71.6 for two A-scans per lane), so they
+// keep the register tables.
+template <int N, int T, int AS, int SCHED>
+constexpr bool twl_fft() {
+    return N == 4096 && T <= 2 && AS == 1 && !(SCHED == Geo<N>::NPASS);
+}
+constexpr int kTwlEntries = 7 * 8 + 7 * 64;
+
+// entry i of the table: pass 1 (i < 56): w^(e*m*64), e = 1 + i/8, m = i%8;
+// pass 2: w^(e*m*8), e = 1 + (i-56)/64, m = (i-56)%64 (w = exp(2 pi i/4096))
+__device__ __forceinline__ int twl_exponent(int i) {
+    return i < 56 ? (1 + i / 8) * (i % 8) * 64 : (1 + (i - 56) / 64) * ((i - 56) % 64) * 8;
 }
 
 // Work a caller runs inside an FFT once the last pass's LDS reads are
@@ -439,13 +468,15 @@ struct NoHook {
 // (pass p writes pa for even p, pb for odd p).
 // Ends without a barrier: the last pass's reads of pa/pb may still be in
 // flight in other wavefronts.
-template <int N, int AS, int SCHED = -1, typename Hook = NoHook>
+template <int N, int AS, int SCHED = -1, typename Hook = NoHook, bool TWL = false>
 __device__ __forceinline__ void stockham_inverse(cf (&v)[AS][8], float2* const (&pa)[AS], float2* const (&pb)[AS],
-                                                 const cf (&tw)[Geo<N>::NPASS][8], Hook&& after_last_reads = {}) {
+                                                 const cf (&tw)[Geo<N>::NPASS][8], Hook&& after_last_reads = {},
+                                                 const float2* twl = nullptr) {
     using G = Geo<N>;
     using P = FftPlan<N, SCHED>;
     constexpr int NP = G::NPASS;
     constexpr int TPA = G::TPA;
+    static_assert(!TWL || twl_fft<N, 1, AS, SCHED>(), "LDS twiddles: N = 4096, one A-scan per lane");
     const int t = threadIdx.x % TPA;
     // lane bases of the unit-stride reads, per padding
     const int rbase0 = pad_slots(t), rbase1 = pad_pass1(t);
@@ -467,11 +498,23 @@ __device__ __forceinline__ void stockham_inverse(cf (&v)[AS][8], float2* const (
                     v[u][e] = {w.x, w.y};
                 }
             }
+            cf twp[8];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) twp[e] = tw[p][e];
+            if (TWL && (p == 1 || p == 2)) {
+                // this pass's twiddles from the workgroup's LDS table
+                const float2* tp = p == 1 ? twl + (t & 7) - 8 : twl + 56 + (t & 63) - 64;
+#pragma unroll
+                for (int e = 1; e < 8; ++e) {
+                    const float2 w = tp[e * (p == 1 ? 8 : 64)];
+                    twp[e] = {w.x, w.y};
+                }
+            }
             if (p == NP - 1) after_last_reads();
 #pragma unroll
             for (int u = 0; u < AS; ++u) {
 #pragma unroll
-                for (int e = RQ; e < 8; ++e) v[u][e] = cmul(v[u][e], tw[p][e]);
+                for (int e = RQ; e < 8; ++e) v[u][e] = cmul(v[u][e], twp[e]);
             }
         }
 #pragma unroll
@@ -1125,6 +1168,10 @@ struct ChainLds {
     // B: the swap schedule's, and the Stockham schedule's with an odd number
     // of passes, the only one with a staging array of its own) are issued
     static constexpr bool DC_PIPE = (MODE & M_DC) != 0 && AS == 1 && !X_ALIAS_B;
+    // twiddles of FFT passes 1 and 2 in LDS (the kernels of T taps that
+    // then fit four waves per SIMD; the DC prologue needs more registers)
+    template <int T>
+    static constexpr bool TWL = twl_fft<N, T, AS, SCHED>() && !PAIR && (MODE & (M_DC | M_STATS)) == 0;
 };
 
 // The chain over A-scans a_begin + k*a_step ... (< a_end) of A, each
@@ -1141,7 +1188,8 @@ struct ChainLds {
 template <int N, int T, int MODE, int IW, int AS, int WG, int SCHED = -1>
 __device__ __forceinline__ void chain_body(const ChainArgs A, float2* bufA, float2* bufB, float* xsep,
                                            typename ChainLds<N, MODE, IW, AS, WG, SCHED>::DcSum* scan_part,
-                                           float* mm_part, long long a_begin, long long a_end, long long a_step) {
+                                           float* mm_part, float2* twl, long long a_begin, long long a_end,
+                                           long long a_step) {
     using G = Geo<N>;
     using L = ChainLds<N, MODE, IW, AS, WG, SCHED>;
     constexpr int TPA = G::TPA;
@@ -1177,13 +1225,25 @@ __device__ __forceinline__ void chain_body(const ChainArgs A, float2* bufA, floa
     // ---- per-lane tables, read once per call
     // tap windows as LDS pointers into the lane's first staging buffer (the
     // other A-scans of the lane sit at constant offsets from it)
+    // (with the LDS twiddles the window starts are kept as 16-bit halves,
+    // two per register: 4 VGPRs fewer for one extraction per output)
+    constexpr bool PACK_TB = L::template TWL<T>;
     const float* tb[8];
+    unsigned tbp[4];
     float tw8[8][T];
     cf ph[8];
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
         const int j = t + e * TPA;
-        tb[e] = xs[0] + gmem(A.tap_base)[j];
+        const int b0 = gmem(A.tap_base)[j];
+        if constexpr (PACK_TB) {
+            if (e & 1)
+                tbp[e >> 1] |= (unsigned)b0 << 16;
+            else
+                tbp[e >> 1] = (unsigned)b0;
+        } else {
+            tb[e] = xs[0] + b0;
+        }
 #pragma unroll
         for (int i = 0; i < T; ++i) tw8[e][i] = gmem(A.tap_w)[j * T + i];
         if (!PAIR && (flags & F_DISP)) {
@@ -1194,10 +1254,20 @@ __device__ __forceinline__ void chain_body(const ChainArgs A, float2* bufA, floa
         }
     }
     cf tw[NP][8];
-    if constexpr (SWAP)
+    constexpr bool TWL = L::template TWL<T>;
+    if constexpr (SWAP) {
         load_twiddles_swap<N>(A.twiddle, t, tw);
-    else
+    } else if constexpr (TWL) {
+        // passes 1 and 2 from the workgroup's LDS table, filled here; the
+        // first FFT reads it after the barrier at the top of the loop
+        for (int i = threadIdx.x; i < kTwlEntries; i += WG) twl[i] = gload2(A.twiddle, twl_exponent(i));
+        load_twiddles<N, SCHED, 3>(A.twiddle, t, tw);
+    } else {
         load_twiddles<N, SCHED>(A.twiddle, t, tw);
+    }
+    // the background constant: per lane for the whole call, or (with the
+    // LDS twiddles, where registers are scarce) read in the epilogue
+    auto bg_const = [&](int k) { return __fadd_rn(__fmul_rn(A.bg_w, gmem(A.bg)[k]), A.bg_off); };
     cf fp[4];
     float bgv[4];
     if constexpr (!STATS) {
@@ -1211,7 +1281,7 @@ __device__ __forceinline__ void chain_body(const ChainArgs A, float2* bufA, floa
             } else {
                 fp[o] = {0.f, 0.f};
             }
-            bgv[o] = (flags & F_BG) ? __fadd_rn(__fmul_rn(A.bg_w, gmem(A.bg)[k]), A.bg_off) : 0.f;
+            bgv[o] = (!TWL && (flags & F_BG)) ? bg_const(k) : 0.f;
         }
     }
 
@@ -1380,7 +1450,8 @@ __device__ __forceinline__ void chain_body(const ChainArgs A, float2* bufA, floa
         for (int u = 0; u < AS; ++u) {
 #pragma unroll
             for (int e = 0; e < 8; ++e) {
-                const float* src = tb[e] + (xs[u] - xs[0]);
+                const float* w0 = PACK_TB ? xs[0] + ((tbp[e >> 1] >> (16 * (e & 1))) & 0xffffu) : tb[e];
+                const float* src = w0 + (xs[u] - xs[0]);
                 float acc = 0.f;
 #pragma unroll
                 for (int i = 0; i < T; ++i) acc = fmaf(tw8[e][i], src[i], acc);
@@ -1437,7 +1508,7 @@ __device__ __forceinline__ void chain_body(const ChainArgs A, float2* bufA, floa
             if constexpr (SWAP)
                 fft_swap<N, AS>(v, pa, pb, tw, next);
             else
-                stockham_inverse<N, AS, SCHED>(v, pa, pb, tw, next);
+                stockham_inverse<N, AS, SCHED, decltype(next)&, TWL>(v, pa, pb, tw, next, twl);
         }
         // after the last pass v[u][e] holds X[t + e*TPA] (twice X for
         // pairs); e < 4 are the kept bins
@@ -1524,7 +1595,11 @@ __device__ __forceinline__ void chain_body(const ChainArgs A, float2* bufA, floa
 #pragma unroll
                     for (int o = 0; o < 4; ++o) {
                         // bgv is 0 without background subtraction: x - 0 == x exactly
-                        __builtin_nontemporal_store(res[o] - bgv[o], dst + o * TPA);
+                        float b = bgv[o];
+                        if constexpr (TWL) {
+                            if (flags & F_BG) b = bg_const(t + o * TPA);
+                        }
+                        __builtin_nontemporal_store(res[o] - b, dst + o * TPA);
                     }
                     // row map: the request's completion stamp, once the
                     // stores are issued (not waited for: the stamp is a
@@ -1565,16 +1640,18 @@ __device__ __forceinline__ void chain_body(const ChainArgs A, float2* bufA, floa
 // few spills measured faster than the one the compiler picks; Lanczos-4
 // (T = 8, 64 weights) does not fit three without spills either.  The FPN
 // statistics modes and the double-precision DC prologue keep the default.
-template <int T, int MODE, int IW>
+template <int N, int T, int MODE, int IW, int AS, int SCHED>
 constexpr int chain_waves_per_eu() {
     if (T >= 8) return 2;
     if ((MODE & M_STATS) || (IW == 4 && (MODE & M_DC))) return OPE_WAVES_PER_EU;
+    // N = 4096 with the LDS twiddles: two 512-lane workgroups per CU
+    if (ChainLds<N, MODE, IW, AS, chain_wg<N>(), SCHED>::template TWL<T>) return 4;
     return 3;
 }
 
 template <int N, int T, int MODE, int IW, int AS = 1, int SCHED = -1>
 __global__ void __launch_bounds__(chain_wg<N>())
-__attribute__((amdgpu_waves_per_eu(chain_waves_per_eu<T, MODE, IW>())))
+__attribute__((amdgpu_waves_per_eu(chain_waves_per_eu<N, T, MODE, IW, AS, SCHED>())))
 chain_kernel(ChainArgs A) {
     constexpr int WG = chain_wg<N>();
     using L = ChainLds<N, MODE, IW, AS, WG, SCHED>;
@@ -1583,6 +1660,7 @@ chain_kernel(ChainArgs A) {
     __shared__ float xsep[L::X];
     __shared__ typename L::DcSum scan_part[L::SCAN];
     __shared__ float mm_part[L::MM];
+    __shared__ float2 twl[L::template TWL<T> ? kTwlEntries : 1];
     constexpr int SLOTS = L::SLOTS;
     long long a_begin, a_step, a_end;
     if constexpr ((MODE & M_STATS) != 0) {
@@ -1601,7 +1679,7 @@ chain_kernel(ChainArgs A) {
             if (A.start && blockIdx.x == 0 && threadIdx.x == 0) *A.start = wall_clock64();
         }
     }
-    chain_body<N, T, MODE, IW, AS, WG, SCHED>(A, bufA, bufB, xsep, scan_part, mm_part, a_begin, a_end, a_step);
+    chain_body<N, T, MODE, IW, AS, WG, SCHED>(A, bufA, bufB, xsep, scan_part, mm_part, twl, a_begin, a_end, a_step);
 }
 
 }  // namespace ope_hip
