@@ -361,9 +361,12 @@ OPE_HIP_API int ope_hip_ragged_create(ope_hip_ctx* const* ctxs, int n_ctx, const
                                       int n_reqs, int chunk_ascans, ope_hip_ragged** out);
 
 /* The whole batch on `stream` (NULL: the HIP null stream): one launch per
- * signal length, in the order of each length's first request.
- * Asynchronous, allocates nothing and may be captured in a graph.  Runs of
- * one batch must not overlap (they share its completion counters). */
+ * signal length, the largest length on `stream` and the others concurrently
+ * on streams of the batch's own, forked from `stream` and joined back into
+ * it by events, so that work queued on `stream` afterwards sees the whole
+ * batch.  Asynchronous, allocates nothing and may be captured in a graph.
+ * Runs of one batch must not overlap (they share its streams and its
+ * completion stamps). */
 OPE_HIP_API int ope_hip_ragged_run(ope_hip_ragged* batch, void* stream);
 
 /* Per-request latency of the last completed run: microseconds from the

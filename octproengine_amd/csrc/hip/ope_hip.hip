@@ -177,6 +177,11 @@ struct ope_hip_ragged {
     RowRef* d_rows = nullptr;
     unsigned long long* d_finish = nullptr;
     unsigned long long* d_start = nullptr;
+    // the lengths after the first run concurrently with it on streams of
+    // their own, forked from and joined back into the caller's stream
+    std::vector<hipStream_t> side;
+    std::vector<hipEvent_t> joins;
+    hipEvent_t fork = nullptr;
 };
 
 // Frames of one context captured into a HIP graph.
@@ -851,6 +856,9 @@ int autofocus_pass(ope_hip_ctx* c, const void* d_in, long long n_ascans, const d
 void free_ragged(ope_hip_ragged* r) {
     if (!r) return;
     (void)hipSetDevice(r->device);
+    for (hipStream_t q : r->side) (void)hipStreamDestroy(q);
+    for (hipEvent_t e : r->joins) (void)hipEventDestroy(e);
+    if (r->fork) (void)hipEventDestroy(r->fork);
     for (void* q : {(void*)r->d_rows, (void*)r->d_finish, (void*)r->d_start})
         if (q) (void)hipFree(q);
     delete r;
@@ -972,6 +980,23 @@ int build_ragged(ope_hip_ragged* r, ope_hip_ctx* const* ctxs, int n_ctx, const o
         k.args.finish = r->d_finish;
         k.args.start = r->classes.empty() ? r->d_start : nullptr;
         r->classes.push_back(k);
+    }
+    // the largest length first: its launch takes the CUs first, and the
+    // smaller lengths' workgroups fill the room it leaves on each CU
+    std::stable_sort(r->classes.begin(), r->classes.end(),
+                     [](const ope_hip_ragged::Class& x, const ope_hip_ragged::Class& y) { return x.ctx->n > y.ctx->n; });
+    if (!r->classes.empty()) {
+        r->classes.front().args.start = r->d_start;
+        for (size_t i = 1; i < r->classes.size(); ++i) r->classes[i].args.start = nullptr;
+    }
+    RAGGED_TRY(hipEventCreateWithFlags(&r->fork, hipEventDisableTiming));
+    for (size_t i = 1; i < r->classes.size(); ++i) {
+        hipStream_t q = nullptr;
+        hipEvent_t e = nullptr;
+        RAGGED_TRY(hipStreamCreateWithFlags(&q, hipStreamNonBlocking));
+        r->side.push_back(q);
+        RAGGED_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        r->joins.push_back(e);
     }
     int khz = 0;
     RAGGED_TRY(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, r->device));
@@ -1908,15 +1933,26 @@ int ope_hip_ragged_run(ope_hip_ragged* r, void* stream) {
     hipError_t e = hipSetDevice(r->device);
     if (e != hipSuccess) return ragged_fail(r, OPE_HIP_ERUNTIME, "hipSetDevice: %s", hipGetErrorString(e));
     const hipStream_t s = static_cast<hipStream_t>(stream);
-    for (const auto& k : r->classes) {
+    // fork: the side streams start after the work the caller queued before
+    e = hipEventRecord(r->fork, s);
+    for (size_t i = 0; e == hipSuccess && i < r->side.size(); ++i) e = hipStreamWaitEvent(r->side[i], r->fork, 0);
+    if (e != hipSuccess) return ragged_fail(r, OPE_HIP_ERUNTIME, "fork: %s", hipGetErrorString(e));
+    for (size_t i = 0; i < r->classes.size(); ++i) {
+        const auto& k = r->classes[i];
         k.ctx->foreign_work = true;
         // the length's default launch (the row-map kernels are built for
         // it), as many workgroups as are resident
         const int mode = M_ROWS | (k.ctx->p.dc_removal_enabled ? M_DC : 0);
         const int rc = launch_variant(k.ctx, k.args, mode, pairable(k.args, mode) ? 2 : 1, default_sched(k.ctx->n),
-                                      0, k.ascans, s);
+                                      0, k.ascans, i == 0 ? s : r->side[i - 1]);
         if (rc) return ragged_fail(r, rc, "length %d: %s", k.ctx->n, k.ctx->err.c_str());
     }
+    // join: the caller's stream continues once every length is done
+    for (size_t i = 0; e == hipSuccess && i < r->side.size(); ++i) {
+        e = hipEventRecord(r->joins[i], r->side[i]);
+        if (e == hipSuccess) e = hipStreamWaitEvent(s, r->joins[i], 0);
+    }
+    if (e != hipSuccess) return ragged_fail(r, OPE_HIP_ERUNTIME, "join: %s", hipGetErrorString(e));
     return OPE_HIP_OK;
 }
 
