@@ -360,13 +360,13 @@ typedef struct ope_hip_ragged ope_hip_ragged;
 OPE_HIP_API int ope_hip_ragged_create(ope_hip_ctx* const* ctxs, int n_ctx, const ope_hip_request* reqs,
                                       int n_reqs, int chunk_ascans, ope_hip_ragged** out);
 
-/* The whole batch on `stream` (NULL: the HIP null stream): one launch per
- * signal length, the largest length on `stream` and the others concurrently
- * on streams of the batch's own, forked from `stream` and joined back into
- * it by events, so that work queued on `stream` afterwards sees the whole
- * batch.  Asynchronous, allocates nothing and may be captured in a graph.
- * Runs of one batch must not overlap (they share its streams and its
- * completion stamps). */
+/* The whole batch on `stream` (NULL: the HIP null stream): the largest
+ * signal length on `stream`, the second largest partly beside it on a
+ * stream of the batch's own (forked from `stream` and joined back into it by
+ * events) and partly after it, the others after them; work queued on
+ * `stream` afterwards sees the whole batch.  Asynchronous, allocates nothing
+ * and may be captured in a graph.  Runs of one batch must not overlap (they
+ * share its stream, its schedule counters and its completion stamps). */
 OPE_HIP_API int ope_hip_ragged_run(ope_hip_ragged* batch, void* stream);
 
 /* Per-request latency of the last completed run: microseconds from the

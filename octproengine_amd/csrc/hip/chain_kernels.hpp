@@ -206,12 +206,22 @@ struct ChainArgs {
     // Once an A-scan's outputs are issued, its first lane raises
     // finish[req] to the wall clock (an atomic max without return, so the
     // wavefront does not wait on it; the clock only grows, so no reset is
-    // needed between runs).  start, where not null, is stamped by workgroup
-    // 0 when it begins.
+    // needed between runs).
     const RowRef* rows;
     unsigned long long* finish;
-    unsigned long long* start;
+    // Dynamic schedule of a row map (tickets not null): workgroups take
+    // chunks of kRowChunk consecutive groups of A-scans from tickets[0],
+    // three at the start and one more each time they enter a new chunk, so
+    // that several launches on the same map (one beside other kernels, one
+    // after them) share the work.  The last of the ticket_wgs workgroups of
+    // those launches to finish clears tickets[0..1] for the next run.
+    unsigned* tickets;
+    unsigned ticket_wgs;
 };
+
+// groups of A-scans per ticket of a dynamic row-map schedule (at least 4:
+// the pipeline looks up to three groups ahead while it knows two chunks)
+constexpr int kRowChunk = 4;
 
 // input classes (numeric values match ope::DataType)
 enum : int { IN_U8 = 1, IN_U16 = 2, IN_U32 = 3, IN_F32 = 9 };
@@ -1318,12 +1328,45 @@ __device__ __forceinline__ void chain_body(const ChainArgs A, float2* bufA, floa
     constexpr bool rowmap = (MODE & M_ROWS) != 0;
     static_assert(!rowmap || !STATS, "row maps are for the throughput mode");
     auto clamp_row = [&](long long a) { return a < a_end ? a : a_end - 1; };
+    // The group of A-scans after group g (workgroup-uniform): g + a_step, or
+    // under a row map's dynamic schedule the next group of g's chunk, else
+    // the first group of the workgroup's next chunk.  cur, nxt and nn are
+    // the workgroup's current chunk and the two it holds tickets for;
+    // callers look at most three groups ahead of the current one.
+    const bool dyn = rowmap && A.tickets != nullptr;
+    constexpr long long CG = (long long)kRowChunk * L::SLOTS;
+    long long cur = 0, nxt = 0, nn = 0;
+    // a ticket passes from lane 0 to the workgroup through the min/max
+    // partials' first slot (free outside the epilogue's reduction, which
+    // ends with a barrier)
+    unsigned* const tk_sh = reinterpret_cast<unsigned*>(mm_part);
+    if constexpr (rowmap) {
+        if (dyn) {
+            if (threadIdx.x == 0) *tk_sh = atomicAdd(A.tickets, 3u);
+            __syncthreads();
+            cur = *tk_sh;
+            nxt = cur + 1;
+            nn = cur + 2;
+            __syncthreads();
+            a_begin = cur * CG;
+            a_step = L::SLOTS;
+        }
+    }
+    auto adv = [&](long long g) -> long long {
+        if (!dyn) return g + a_step;
+        const long long ch = g / CG;
+        if (g + a_step < (ch + 1) * CG) return g + a_step;  // a_step = SLOTS here
+        return (ch == cur ? nxt : nn) * CG;
+    };
     const void* nin[AS];
     uint32_t raw[AS][8];
-    auto load_next = [&](int u, long long an) {
+    // loads group g's samples into raw (and reads the input rows of group
+    // adv(g) under a row map)
+    auto load_next = [&](int u, long long g) {
+        const long long an = g + slot * AS + u;
         if constexpr (rowmap) {
             load_raw<IW>(nin[u], N, 0, t, raw[u]);
-            nin[u] = gmem(A.rows)[clamp_row(an + a_step)].in;
+            nin[u] = gmem(A.rows)[clamp_row(adv(g) + slot * AS + u)].in;
         } else {
             if (an < a_end) load_raw<IW>(A.in, N, an, t, raw[u]);
         }
@@ -1332,10 +1375,9 @@ __device__ __forceinline__ void chain_body(const ChainArgs A, float2* bufA, floa
     for (int u = 0; u < AS; ++u) {
 #pragma unroll
         for (int i = 0; i < 8; ++i) raw[u][i] = 0;
-        const long long a = a_begin + slot * AS + u;
         nin[u] = nullptr;
-        if constexpr (rowmap) nin[u] = gmem(A.rows)[clamp_row(a)].in;
-        load_next(u, a);
+        if constexpr (rowmap) nin[u] = gmem(A.rows)[clamp_row(a_begin + slot * AS + u)].in;
+        load_next(u, a_begin);
     }
 
     // Staging of A-scan group a: convert the raw words (loaded one group
@@ -1385,7 +1427,7 @@ __device__ __forceinline__ void chain_body(const ChainArgs A, float2* bufA, floa
                 dst[0] = make_float4(x8[0], x8[1], x8[2], x8[3]);
                 dst[1] = make_float4(x8[4], x8[5], x8[6], x8[7]);
             }
-            load_next(u, a + a_step + slot * AS + u);
+            load_next(u, adv(a));
         }
     };
     if constexpr (PIPE) {
@@ -1393,7 +1435,35 @@ __device__ __forceinline__ void chain_body(const ChainArgs A, float2* bufA, floa
         stage_rest(a_begin);
     }
 
-    for (long long a0 = a_begin; a0 < a_end; a0 += a_step) {
+    unsigned tk = 0;        // lane 0: the ticket requested on entering the current chunk
+    int since_ticket = -1;  // iterations since that request (-1: none pending)
+    // the next group; on entering a new chunk the workgroup moves to the
+    // chunks it holds tickets for and lane 0 requests one more
+    auto adv_and_step = [&](long long a0) -> long long {
+        const long long na = adv(a0);
+        if constexpr (rowmap) {
+            if (dyn && na / CG != a0 / CG) {
+                cur = nxt;
+                nxt = nn;
+                if (threadIdx.x == 0) tk = atomicAdd(A.tickets, 1u);
+                since_ticket = 0;
+            }
+        }
+        return na;
+    };
+    for (long long a0 = a_begin; a0 < a_end; a0 = adv_and_step(a0)) {
+        if constexpr (rowmap) {
+            // the ticket requested on entering this chunk: all lanes read it
+            // one iteration later (barriers around the slot: the waves of an
+            // A-scan of up to 64 lanes meet at no other workgroup barrier)
+            if (dyn && since_ticket >= 0 && ++since_ticket == 2) {
+                if (threadIdx.x == 0) *tk_sh = tk;
+                __syncthreads();
+                nn = *tk_sh;
+                __syncthreads();
+                since_ticket = -1;
+            }
+        }
         // the staging shares buffer B, which the previous A-scan's last FFT
         // pass may still be reading in other wavefronts
         if constexpr (L::SWAP_TOP_BARRIER) __syncthreads();
@@ -1420,7 +1490,7 @@ __device__ __forceinline__ void chain_body(const ChainArgs A, float2* bufA, floa
                     dst[1] = make_float4(xv[u][4], xv[u][5], xv[u][6], xv[u][7]);
                 }
                 // prefetch the next A-scan while this one is processed
-                load_next(u, a0 + a_step + slot * AS + u);
+                load_next(u, adv(a0));
             }
         }
 
@@ -1465,11 +1535,11 @@ __device__ __forceinline__ void chain_body(const ChainArgs A, float2* bufA, floa
         // again, so that its loads are unconditional)
         auto next = [&]() {
             if constexpr (PIPE) {
-                if (rowmap || a0 + a_step < a_end) {
+                if (rowmap || adv(a0) < a_end) {
                     if constexpr (SPLIT)
                         stage_prefix();
                     else
-                        stage_rest(a0 + a_step);
+                        stage_rest(adv(a0));
                 }
             }
         };
@@ -1609,7 +1679,20 @@ __device__ __forceinline__ void chain_body(const ChainArgs A, float2* bufA, floa
             }
         }
         if constexpr (PIPE && SPLIT) {
-            if (rowmap || a0 + a_step < a_end) stage_rest(a0 + a_step);
+            if (rowmap || adv(a0) < a_end) stage_rest(adv(a0));
+        }
+    }
+
+    if constexpr (rowmap) {
+        // the last workgroup of the launches sharing the tickets to stop
+        // taking them clears the counters for the next run (its own last
+        // request is complete before it counts itself)
+        if (dyn && threadIdx.x == 0) {
+            __threadfence();
+            if (atomicAdd(A.tickets + 1, 1u) == A.ticket_wgs - 1) {
+                atomicExch(A.tickets, 0u);
+                atomicExch(A.tickets + 1, 0u);
+            }
         }
     }
 
@@ -1675,9 +1758,6 @@ chain_kernel(ChainArgs A) {
         a_begin = (long long)blockIdx.x * SLOTS;
         a_end = A.n_ascans;
         a_step = (long long)gridDim.x * SLOTS;
-        if constexpr ((MODE & M_ROWS) != 0) {
-            if (A.start && blockIdx.x == 0 && threadIdx.x == 0) *A.start = wall_clock64();
-        }
     }
     chain_body<N, T, MODE, IW, AS, WG, SCHED>(A, bufA, bufB, xsep, scan_part, mm_part, twl, a_begin, a_end, a_step);
 }

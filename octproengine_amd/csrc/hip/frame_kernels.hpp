@@ -112,6 +112,9 @@ __global__ void fill_zero_f64(double* __restrict__ p, long long n) {
     if (i < n) p[i] = 0.0;
 }
 
+// The wall clock into *p (one lane): a stream-ordered time stamp.
+__global__ void stamp_clock_kernel(unsigned long long* p) { *p = wall_clock64(); }
+
 // Column mean of the processed frame (cpu_backend.cpp:105-118), accumulated
 // in double.
 __global__ void bg_record_kernel(const float* __restrict__ out, long long rows, int width,

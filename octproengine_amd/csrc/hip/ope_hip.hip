@@ -172,6 +172,9 @@ struct ope_hip_ragged {
         ope_hip_ctx* ctx;
         ChainArgs args;      // the row map of this length's A-scans
         long long ascans;
+        ChainLaunch launch;  // the length's default launch (row-map kernel)
+        long long grid_side = 0, grid_main = 0;  // workgroups beside the largest length and after it
+        unsigned* d_tickets = nullptr;           // [2] the dynamic schedule's counters
     };
     std::vector<Class> classes;  // in the order of their first request
     RowRef* d_rows = nullptr;
@@ -856,6 +859,8 @@ int autofocus_pass(ope_hip_ctx* c, const void* d_in, long long n_ascans, const d
 void free_ragged(ope_hip_ragged* r) {
     if (!r) return;
     (void)hipSetDevice(r->device);
+    for (const auto& k : r->classes)
+        if (k.d_tickets) (void)hipFree(k.d_tickets);
     for (hipStream_t q : r->side) (void)hipStreamDestroy(q);
     for (hipEvent_t e : r->joins) (void)hipEventDestroy(e);
     if (r->fork) (void)hipEventDestroy(r->fork);
@@ -978,19 +983,43 @@ int build_ragged(ope_hip_ragged* r, ope_hip_ctx* const* ctxs, int n_ctx, const o
         k.args.n_ascans = k.ascans;
         k.args.rows = r->d_rows + cls_first[cls];
         k.args.finish = r->d_finish;
-        k.args.start = r->classes.empty() ? r->d_start : nullptr;
         r->classes.push_back(k);
     }
     // the largest length first: its launch takes the CUs first, and the
     // smaller lengths' workgroups fill the room it leaves on each CU
     std::stable_sort(r->classes.begin(), r->classes.end(),
                      [](const ope_hip_ragged::Class& x, const ope_hip_ragged::Class& y) { return x.ctx->n > y.ctx->n; });
-    if (!r->classes.empty()) {
-        r->classes.front().args.start = r->d_start;
-        for (size_t i = 1; i < r->classes.size(); ++i) r->classes[i].args.start = nullptr;
+    // the launches: the largest length on the caller's stream with as many
+    // workgroups as are resident, striding over its A-scans; the second
+    // largest twice on one dynamic schedule: beside it on a side stream with
+    // one workgroup per CU (so that whichever the GPU dispatches first, each
+    // CU keeps room for a workgroup of the largest length), and after it on
+    // the caller's stream with as many workgroups as are resident, which
+    // take what the side launch has left; the others after those, striding.
+    // (Side launches of the third length too, after the second's, slowed
+    // the largest length's launch more than they gained: 1.73-1.76 ms
+    // against 1.61-1.65 ms for the 512-request batch.)
+    for (size_t i = 0; i < r->classes.size(); ++i) {
+        auto& k = r->classes[i];
+        const int mode0 = M_ROWS | (k.ctx->p.dc_removal_enabled ? M_DC : 0);
+        const int as = pairable(k.args, mode0) ? 2 : 1;
+        k.launch = pick_kernel(k.ctx->n, k.ctx->taps, variant_mode(k.args, mode0, as),
+                               (int)input_bytes(k.ctx->p.input_type), as, default_sched(k.ctx->n));
+        if (!k.launch.fn) return ragged_fail(r, OPE_HIP_EUNSUP, "length %d: row-map kernel not built", k.ctx->n);
+        int per_cu = 0;
+        RAGGED_TRY(cached_occupancy(k.ctx, reinterpret_cast<const void*>(k.launch.fn), k.launch.wg, &per_cu));
+        const long long groups = (k.ascans + k.launch.slots - 1) / k.launch.slots;
+        k.grid_main = std::min<long long>(groups, (long long)per_cu * k.ctx->cu_count);
+        if (i == 1) {
+            k.grid_side = std::min<long long>(groups, k.ctx->cu_count);
+            RAGGED_TRY(hipMalloc(&k.d_tickets, 2 * sizeof(unsigned)));
+            RAGGED_TRY(hipMemset(k.d_tickets, 0, 2 * sizeof(unsigned)));
+            k.args.tickets = k.d_tickets;
+            k.args.ticket_wgs = (unsigned)(k.grid_side + k.grid_main);
+        }
     }
     RAGGED_TRY(hipEventCreateWithFlags(&r->fork, hipEventDisableTiming));
-    for (size_t i = 1; i < r->classes.size(); ++i) {
+    if (r->classes.size() > 1) {
         hipStream_t q = nullptr;
         hipEvent_t e = nullptr;
         RAGGED_TRY(hipStreamCreateWithFlags(&q, hipStreamNonBlocking));
@@ -1933,20 +1962,22 @@ int ope_hip_ragged_run(ope_hip_ragged* r, void* stream) {
     hipError_t e = hipSetDevice(r->device);
     if (e != hipSuccess) return ragged_fail(r, OPE_HIP_ERUNTIME, "hipSetDevice: %s", hipGetErrorString(e));
     const hipStream_t s = static_cast<hipStream_t>(stream);
-    // fork: the side streams start after the work the caller queued before
+    // the run's start (the latencies' origin), then the fork: the side
+    // streams start after it and after the work the caller queued before
+    hipLaunchKernelGGL(stamp_clock_kernel, dim3(1), dim3(1), 0, s, r->d_start);
     e = hipEventRecord(r->fork, s);
     for (size_t i = 0; e == hipSuccess && i < r->side.size(); ++i) e = hipStreamWaitEvent(r->side[i], r->fork, 0);
     if (e != hipSuccess) return ragged_fail(r, OPE_HIP_ERUNTIME, "fork: %s", hipGetErrorString(e));
-    for (size_t i = 0; i < r->classes.size(); ++i) {
-        const auto& k = r->classes[i];
-        k.ctx->foreign_work = true;
-        // the length's default launch (the row-map kernels are built for
-        // it), as many workgroups as are resident
-        const int mode = M_ROWS | (k.ctx->p.dc_removal_enabled ? M_DC : 0);
-        const int rc = launch_variant(k.ctx, k.args, mode, pairable(k.args, mode) ? 2 : 1, default_sched(k.ctx->n),
-                                      0, k.ascans, i == 0 ? s : r->side[i - 1]);
-        if (rc) return ragged_fail(r, rc, "length %d: %s", k.ctx->n, k.ctx->err.c_str());
+    // (grids and the split between the launches: build_ragged)
+    for (const auto& k : r->classes) k.ctx->foreign_work = true;
+    if (r->classes.size() > 1) {
+        const auto& k = r->classes[1];
+        hipLaunchKernelGGL(k.launch.fn, dim3((unsigned)k.grid_side), dim3(k.launch.wg), 0, r->side[0], k.args);
     }
+    for (const auto& k : r->classes)
+        hipLaunchKernelGGL(k.launch.fn, dim3((unsigned)k.grid_main), dim3(k.launch.wg), 0, s, k.args);
+    e = hipGetLastError();
+    if (e != hipSuccess) return ragged_fail(r, OPE_HIP_ERUNTIME, "ragged launch: %s", hipGetErrorString(e));
     // join: the caller's stream continues once every length is done
     for (size_t i = 0; e == hipSuccess && i < r->side.size(); ++i) {
         e = hipEventRecord(r->joins[i], r->side[i]);
