@@ -334,7 +334,10 @@ OPE_HIP_API int ope_hip_op_fpn_removal(ope_hip_ctx* ctx, const float* input, con
  * signal lengths and A-scan counts in one call: one persistent launch per
  * signal length present, each that length's own chain kernel (its tuned
  * launch variant and occupancy) over a row map of the requests' A-scans,
- * for serving callers that queue requests from several scan protocols. */
+ * for serving callers that queue requests from several scan protocols.
+ * Every request's output equals that of a process_device call over its
+ * A-scans alone, bit for bit (without dispersion and DC removal, where two
+ * A-scans share one FFT, each request starts a new pair). */
 
 /* One request: n_ascans A-scans of signal_length samples from d_in (device,
  * [n_ascans][signal_length] in the contexts' input type, 16-byte aligned)

@@ -1368,7 +1368,10 @@ __device__ __forceinline__ void chain_body(const ChainArgs A, float2* bufA, floa
             load_raw<IW>(nin[u], N, 0, t, raw[u]);
             nin[u] = gmem(A.rows)[clamp_row(adv(g) + slot * AS + u)].in;
         } else {
-            if (an < a_end) load_raw<IW>(A.in, N, an, t, raw[u]);
+            // pairs: a slot whose second A-scan lies past the end pairs its
+            // first with itself (as a row map pads a request of odd count)
+            const long long src = (PAIR && u == 1 && an == a_end) ? an - 1 : an;
+            if (src < a_end) load_raw<IW>(A.in, N, src, t, raw[u]);
         }
     };
 #pragma unroll

@@ -180,6 +180,7 @@ struct ope_hip_ragged {
     RowRef* d_rows = nullptr;
     unsigned long long* d_finish = nullptr;
     unsigned long long* d_start = nullptr;
+    float* d_discard = nullptr;  // output of the rows that pad paired requests
     // the lengths after the first run concurrently with it on streams of
     // their own, forked from and joined back into the caller's stream
     std::vector<hipStream_t> side;
@@ -864,7 +865,7 @@ void free_ragged(ope_hip_ragged* r) {
     for (hipStream_t q : r->side) (void)hipStreamDestroy(q);
     for (hipEvent_t e : r->joins) (void)hipEventDestroy(e);
     if (r->fork) (void)hipEventDestroy(r->fork);
-    for (void* q : {(void*)r->d_rows, (void*)r->d_finish, (void*)r->d_start})
+    for (void* q : {(void*)r->d_rows, (void*)r->d_finish, (void*)r->d_start, (void*)r->d_discard})
         if (q) (void)hipFree(q);
     delete r;
 }
@@ -923,6 +924,15 @@ int build_ragged(ope_hip_ragged* r, ope_hip_ctx* const* ctxs, int n_ctx, const o
     // their first request
     std::vector<int> req_cls(n_reqs, -1), order;
     std::vector<long long> cls_ascans(n_ctx, 0);
+    // Without dispersion and DC removal two A-scans share one FFT (rows 2k
+    // and 2k+1 of a launch).  Each request of such a length then starts on
+    // an even row, and one of odd count ends with its last A-scan again,
+    // written to a discard row: its A-scans pair as in a frame launch of
+    // the request alone (whose last odd A-scan pairs with itself), so the
+    // outputs agree bit for bit.
+    std::vector<char> paired(n_ctx, 0);
+    for (int i = 0; i < n_ctx; ++i)
+        paired[i] = pairable(steady_args(ctxs[i]), ctxs[i]->p.dc_removal_enabled ? M_DC : 0);
     for (int q = 0; q < n_reqs; ++q) {
         const ope_hip_request& rq = reqs[q];
         int cls = -1;
@@ -937,7 +947,7 @@ int build_ragged(ope_hip_ragged* r, ope_hip_ctx* const* ctxs, int n_ctx, const o
         req_cls[q] = cls;
         r->req_ascans[q] = rq.n_ascans;
         if (rq.n_ascans > 0 && cls_ascans[cls] == 0) order.push_back(cls);
-        cls_ascans[cls] += rq.n_ascans;
+        cls_ascans[cls] += rq.n_ascans + (paired[cls] ? (rq.n_ascans & 1) : 0);
     }
     // the row map: each class's A-scans contiguous, requests in order
     long long total = 0;
@@ -948,14 +958,23 @@ int build_ragged(ope_hip_ragged* r, ope_hip_ctx* const* ctxs, int n_ctx, const o
     }
     std::vector<RowRef> rows((size_t)total);
     std::vector<long long> fill(cls_first);
+    std::vector<size_t> pads;  // rows whose output is the discard row
+    int pad_n = 0;
     for (int q = 0; q < n_reqs; ++q) {
         const ope_hip_request& rq = reqs[q];
-        const size_t iw = input_bytes(ctxs[req_cls[q]]->p.input_type);
+        const int cls = req_cls[q];
+        const size_t iw = input_bytes(ctxs[cls]->p.input_type);
         for (long long a = 0; a < rq.n_ascans; ++a) {
-            RowRef& e = rows[(size_t)fill[req_cls[q]]++];
+            RowRef& e = rows[(size_t)fill[cls]++];
             e.in = static_cast<const char*>(rq.d_in) + (size_t)a * rq.signal_length * iw;
             e.out = rq.d_out + (size_t)a * (rq.signal_length / 2);
             e.req = q;
+        }
+        if (paired[cls] && (rq.n_ascans & 1)) {
+            const size_t i = (size_t)fill[cls]++;
+            rows[i] = rows[i - 1];
+            pads.push_back(i);
+            pad_n = std::max(pad_n, rq.signal_length);
         }
     }
 
@@ -971,6 +990,10 @@ int build_ragged(ope_hip_ragged* r, ope_hip_ctx* const* ctxs, int n_ctx, const o
     RAGGED_TRY(hipMalloc(&r->d_rows, std::max<size_t>(rows.size(), 1) * sizeof(RowRef)));
     RAGGED_TRY(hipMalloc(&r->d_finish, nq * sizeof(unsigned long long)));
     RAGGED_TRY(hipMalloc(&r->d_start, sizeof(unsigned long long)));
+    if (!pads.empty()) {
+        RAGGED_TRY(hipMalloc(&r->d_discard, (size_t)(pad_n / 2) * sizeof(float)));
+        for (size_t i : pads) rows[i].out = r->d_discard;
+    }
     if (!rows.empty())
         RAGGED_TRY(hipMemcpy(r->d_rows, rows.data(), rows.size() * sizeof(RowRef), hipMemcpyHostToDevice));
     std::vector<unsigned long long> zt(nq, 0ull);

@@ -96,6 +96,29 @@ def test_pairs_tiny_frames(a):
     assert db_error(pair, ref) <= DB_TOL and db_lin_error(pair, ref) <= LIN_TOL
 
 
+@pytest.mark.parametrize("n", [256, 2048])
+def test_pairs_partners_fixed_by_position(n):
+    """A-scans 2k and 2k+1 of a launch share one FFT and the last A-scan of
+    an odd count pairs with itself, so each output depends only on its own
+    pair: the frame split at an even A-scan, or cut to its last A-scan,
+    gives the same floats bit for bit."""
+    from octproengine_amd import _capi
+    a = 2 * 37 + 1
+    p = db_params(_params(n, a, "cubic"))
+    raw = S.fringes(n, a, seed=19)
+    t = standard_tables(n)
+    g = GpuChain(p, t.curve, t.window, t.phasor)
+    g.ctx.set_sample_offset(_capi.OFFSET_FIXED, 20000.0)
+    whole = g.frame(raw)
+    head = g.frame(raw[:a - 1].copy())
+    tail = g.frame(raw[a - 3:].copy())  # one pair and the odd A-scan
+    last = g.frame(raw[a - 1:].copy())
+    g.close()
+    assert np.array_equal(whole[:a - 1], head)
+    assert np.array_equal(whole[a - 3:], tail)
+    assert np.array_equal(whole[a - 1:], last)
+
+
 def test_pairs_fpn_profile_auto_min_max_background_flip():
     """The epilogue steps on split spectra: FPN subtraction (the profile is
     applied to twice the spectrum, scaled by two), per-A-scan auto min/max,

@@ -221,3 +221,48 @@ def test_ragged_empty_batches_and_graph_replay():
     batch.close()
     for c in chains.values():
         c[0].close()
+
+
+# The second-largest length runs on the dynamic chunk schedule shared by two
+# launches: lengths of up to 64 lanes per A-scan (several A-scans per
+# workgroup iteration), real pairs (no dispersion, two A-scans per lane) and
+# DC removal take it as well as the 1024/2048-point single FFTs.
+@pytest.mark.parametrize("lengths,disp,dc", [((512, 256), True, False), ((4096, 1024), False, False),
+                                             ((2048, 512), True, True), ((4096, 2048, 256), False, True)])
+def test_ragged_dynamic_schedule_shapes(lengths, disp, dc):
+    import torch
+    from octproengine_amd import _capi
+    rng = np.random.default_rng(31)
+    chains = {}
+    for n in lengths:
+        p = db_params(O.ChainParams(signal_length=n, ascans_per_bscan=64, bscans_per_buffer=1, resampling=True,
+                                    interpolation=O.CUBIC, windowing=True, dispersion=disp, dc_removal=dc,
+                                    dc_window=32))
+        t = standard_tables(n)
+        g = GpuChain(p, t.curve, t.window, t.phasor)
+        g.ctx.set_sample_offset(_capi.OFFSET_FIXED, 20000.0)
+        chains[n] = (g, p, t)
+    reqs = make_requests(rng, lengths, 30, 700)
+    print(f"[ragged] lengths {lengths} disp={disp} dc={dc}: {sum(r['a'] for r in reqs)} A-scans", flush=True)
+    batch = run_batch(chains, reqs)
+    for rep in range(2):  # the schedule's counters clear themselves between runs
+        if rep:
+            for r in reqs:
+                r["d_out"].fill_(float("nan"))
+            batch.run(torch.cuda.current_stream().cuda_stream)
+            torch.cuda.synchronize()
+        for i, r in enumerate(reqs):
+            if r["a"] == 0:
+                continue
+            got = r["d_out"].cpu().numpy()
+            chain, p, t = chains[r["n"]]
+            assert np.array_equal(got, chain.frame(r["raw"])), (rep, i, r["n"], r["a"])
+            if rep == 0 and i % 5 == 0:
+                o = O.process_frame(r["raw"], p, t.curve, t.window, t.phasor)
+                assert db_error(got, o) <= DB_TOL
+    lat = batch.latencies_us()
+    nonempty = np.array([r["a"] > 0 for r in reqs])
+    assert (lat[nonempty] > 0).all() and (lat[~nonempty] == 0).all()
+    batch.close()
+    for c in chains.values():
+        c[0].close()
