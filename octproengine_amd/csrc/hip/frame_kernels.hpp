@@ -115,6 +115,14 @@ __global__ void fill_zero_f64(double* __restrict__ p, long long n) {
 // The wall clock into *p (one lane): a stream-ordered time stamp.
 __global__ void stamp_clock_kernel(unsigned long long* p) { *p = wall_clock64(); }
 
+// Holds its stream for `ticks` of the wall clock (one wave, sleeping), so
+// that work queued behind it reaches the GPU after work queued at the same
+// time on another stream.  Ends by itself.
+__global__ void hold_clock_kernel(unsigned long long ticks) {
+    const unsigned long long t0 = wall_clock64();
+    while (wall_clock64() - t0 < ticks) __builtin_amdgcn_s_sleep(8);
+}
+
 // Column mean of the processed frame (cpu_backend.cpp:105-118), accumulated
 // in double.
 __global__ void bg_record_kernel(const float* __restrict__ out, long long rows, int width,

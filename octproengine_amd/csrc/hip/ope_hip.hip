@@ -174,6 +174,7 @@ struct ope_hip_ragged {
         long long ascans;
         ChainLaunch launch;  // the length's default launch (row-map kernel)
         long long grid_side = 0, grid_main = 0;  // workgroups beside the largest length and after it
+        unsigned side_lds = 0;                   // dynamic LDS of the side launch (bytes)
         unsigned* d_tickets = nullptr;           // [2] the dynamic schedule's counters
     };
     std::vector<Class> classes;  // in the order of their first request
@@ -1035,6 +1036,27 @@ int build_ragged(ope_hip_ragged* r, ope_hip_ctx* const* ctxs, int n_ctx, const o
         k.grid_main = std::min<long long>(groups, (long long)per_cu * k.ctx->cu_count);
         if (i == 1) {
             k.grid_side = std::min<long long>(groups, k.ctx->cu_count);
+            // One side workgroup per CU whichever launch the GPU dispatches
+            // first: padded with unused LDS to more than half a CU's, two
+            // cannot share a CU (two there would leave no room for a
+            // workgroup of the largest length), while one still fits beside
+            // the largest length's resident workgroups.  Where those leave
+            // half a CU or less, no pad.
+            const auto& big = r->classes[0];
+            int lds_cu = 0, big_cu = 0;
+            hipFuncAttributes fa_big{}, fa_side{};
+            RAGGED_TRY(hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor,
+                                             r->device));
+            RAGGED_TRY(cached_occupancy(big.ctx, reinterpret_cast<const void*>(big.launch.fn), big.launch.wg, &big_cu));
+            RAGGED_TRY(hipFuncGetAttributes(&fa_big, reinterpret_cast<const void*>(big.launch.fn)));
+            RAGGED_TRY(hipFuncGetAttributes(&fa_side, reinterpret_cast<const void*>(k.launch.fn)));
+            const long long room = (long long)lds_cu - (long long)big_cu * (long long)fa_big.sharedSizeBytes;
+            const long long half = lds_cu / 2;
+            if (room > half && room >= (long long)fa_side.sharedSizeBytes) {
+                const long long want = half + (room - half) / 2;  // between half a CU and the room left
+                if (want > (long long)fa_side.sharedSizeBytes)
+                    k.side_lds = (unsigned)(want - (long long)fa_side.sharedSizeBytes);
+            }
             RAGGED_TRY(hipMalloc(&k.d_tickets, 2 * sizeof(unsigned)));
             RAGGED_TRY(hipMemset(k.d_tickets, 0, 2 * sizeof(unsigned)));
             k.args.tickets = k.d_tickets;
@@ -1994,8 +2016,15 @@ int ope_hip_ragged_run(ope_hip_ragged* r, void* stream) {
     // (grids and the split between the launches: build_ragged)
     for (const auto& k : r->classes) k.ctx->foreign_work = true;
     if (r->classes.size() > 1) {
+        // the side launch 20 us behind the largest length's: the GPU then
+        // dispatches the largest length's workgroups first, and their waves,
+        // the older ones on each SIMD, keep the issue priority (the largest
+        // length is the batch's critical path)
+        hipLaunchKernelGGL(hold_clock_kernel, dim3(1), dim3(64), 0, r->side[0],
+                           (unsigned long long)(20e-6 * r->tick_hz));
         const auto& k = r->classes[1];
-        hipLaunchKernelGGL(k.launch.fn, dim3((unsigned)k.grid_side), dim3(k.launch.wg), 0, r->side[0], k.args);
+        hipLaunchKernelGGL(k.launch.fn, dim3((unsigned)k.grid_side), dim3(k.launch.wg), k.side_lds, r->side[0],
+                           k.args);
     }
     for (const auto& k : r->classes)
         hipLaunchKernelGGL(k.launch.fn, dim3((unsigned)k.grid_main), dim3(k.launch.wg), 0, s, k.args);
