@@ -2011,23 +2011,29 @@ int ope_hip_ragged_run(ope_hip_ragged* r, void* stream) {
     // streams start after it and after the work the caller queued before
     hipLaunchKernelGGL(stamp_clock_kernel, dim3(1), dim3(1), 0, s, r->d_start);
     e = hipEventRecord(r->fork, s);
-    for (size_t i = 0; e == hipSuccess && i < r->side.size(); ++i) e = hipStreamWaitEvent(r->side[i], r->fork, 0);
     if (e != hipSuccess) return ragged_fail(r, OPE_HIP_ERUNTIME, "fork: %s", hipGetErrorString(e));
     // (grids and the split between the launches: build_ragged)
     for (const auto& k : r->classes) k.ctx->foreign_work = true;
+    // The largest length first, before the side stream's work is queued:
+    // the GPU then dispatches its workgroups first even while this call is
+    // still queueing the rest, and their waves, the older ones on each
+    // SIMD, keep the issue priority (the largest length is the batch's
+    // critical path).  The side launch also waits 20 us behind the fork.
+    const auto& big = r->classes[0];
+    hipLaunchKernelGGL(big.launch.fn, dim3((unsigned)big.grid_main), dim3(big.launch.wg), 0, s, big.args);
+    for (size_t i = 0; e == hipSuccess && i < r->side.size(); ++i) e = hipStreamWaitEvent(r->side[i], r->fork, 0);
+    if (e != hipSuccess) return ragged_fail(r, OPE_HIP_ERUNTIME, "fork: %s", hipGetErrorString(e));
     if (r->classes.size() > 1) {
-        // the side launch 20 us behind the largest length's: the GPU then
-        // dispatches the largest length's workgroups first, and their waves,
-        // the older ones on each SIMD, keep the issue priority (the largest
-        // length is the batch's critical path)
         hipLaunchKernelGGL(hold_clock_kernel, dim3(1), dim3(64), 0, r->side[0],
                            (unsigned long long)(20e-6 * r->tick_hz));
         const auto& k = r->classes[1];
         hipLaunchKernelGGL(k.launch.fn, dim3((unsigned)k.grid_side), dim3(k.launch.wg), k.side_lds, r->side[0],
                            k.args);
     }
-    for (const auto& k : r->classes)
+    for (size_t i = 1; i < r->classes.size(); ++i) {
+        const auto& k = r->classes[i];
         hipLaunchKernelGGL(k.launch.fn, dim3((unsigned)k.grid_main), dim3(k.launch.wg), 0, s, k.args);
+    }
     e = hipGetLastError();
     if (e != hipSuccess) return ragged_fail(r, OPE_HIP_ERUNTIME, "ragged launch: %s", hipGetErrorString(e));
     // join: the caller's stream continues once every length is done
