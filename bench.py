@@ -141,7 +141,7 @@ def ragged_workload(seed=0, count=512):
     return [(int(n), int(a)) for n, a in zip(lengths, ascans)]
 
 
-def ragged_measurement(capi, S, torch, stream, reps=3):
+def ragged_measurement(capi, S, torch, stream, reps=5):
     """The mixed-length batch, one persistent launch per signal length over
     a row map of its requests; A-scans/s of the whole batch and the p50/p99
     per-request latency from the start of the first launch, plus the same

@@ -337,7 +337,9 @@ OPE_HIP_API int ope_hip_op_fpn_removal(ope_hip_ctx* ctx, const float* input, con
  * for serving callers that queue requests from several scan protocols.
  * Every request's output equals that of a process_device call over its
  * A-scans alone, bit for bit (without dispersion and DC removal, where two
- * A-scans share one FFT, each request starts a new pair). */
+ * A-scans share one FFT, each request starts a new pair).  Within a signal
+ * length the requests are processed shortest first (fewest A-scans), which
+ * minimises the mean request latency. */
 
 /* One request: n_ascans A-scans of signal_length samples from d_in (device,
  * [n_ascans][signal_length] in the contexts' input type, 16-byte aligned)
