@@ -624,6 +624,33 @@ struct SwapFft {
     static constexpr int e3_write_off(int e) { return 8 * (e % RQ) + 64 * (e / RQ); }
 };
 
+// The swap schedule at N = 1024 and 2048 with 8 taps (Lanczos-4), one A-scan
+// per lane: the twiddles of its passes 1 and 2 depend on r1 = 8 d2 + r2
+// (64 values) and r2 (8 values) only, so the workgroup keeps them once in
+// LDS (the same floats as the global table, so results are bit-identical)
+// and each lane reads its 7 + (8 - RQ) there next to the passes' data
+// reads.  That frees the 2 x 7 complex registers those passes held, and
+// the Lanczos-4 kernels fit three waves per SIMD instead of two (2048-point
+// frames 65.5 -> 63.4 us, with DC removal 95.5 -> 88.5, the 2048 x 1024 x
+// 256 volume 1.42-1.48 -> 1.33 ms).  Lanczos-8 stays at two waves with the
+// registers freed and ran slower with the table (80.7 -> 87.6 us), so it
+// keeps its register twiddles.
+template <int N, int T, int AS, int SCHED>
+constexpr bool twl_swap() {
+    return (N == 1024 || N == 2048) && SCHED == Geo<N>::NPASS && AS == 1 && T == 8;
+}
+
+// entry i of that table: pass 1 (i < 448): w^(8 r1 e), e = 1 + i/64,
+// r1 = i%64; pass 2: w^(64 r2 (e/RQ)), e = RQ + (i-448)/8, r2 = (i-448)%8
+// (w = exp(2 pi i/N); entries past pass 2's are not read)
+template <int N>
+__device__ __forceinline__ int twl_swap_exponent(int i) {
+    constexpr int RQ = SwapFft<N>::RQ;
+    if (i < 448) return (8 * (i % 64) * (1 + i / 64)) & (N - 1);
+    const int j = i - 448;
+    return (64 * (j % 8) * ((RQ + j / 8) / RQ)) & (N - 1);
+}
+
 // ------------------------------------------------------------------ N = 4096 with a register exchange
 //
 // Radices 8, 8, 8, 8 with n = 512 d0 + 64 d1 + 8 d2 + r2 and
@@ -780,12 +807,32 @@ __device__ __forceinline__ void fft_swap4096(cf (&v)[AS][8], float2* const (&pa)
     for (int u = 0; u < AS; ++u) idft8(v[u]);
 }
 
-// The swap schedule of N = 1024 and 2048 (see SwapFft).
-template <int N, int AS, typename Hook>
+// The swap schedule of N = 1024 and 2048 (see SwapFft); TWLS: the twiddles
+// of passes 1 and 2 from the workgroup's LDS table twl (twl_swap).
+template <int N, int AS, typename Hook, bool TWLS = false>
 __device__ __forceinline__ void fft_swap_small(cf (&v)[AS][8], float2* const (&pa)[AS], float2* const (&pb)[AS],
-                                               const cf (&tw)[Geo<N>::NPASS][8], Hook&& after_last_reads) {
+                                               const cf (&tw)[Geo<N>::NPASS][8], Hook&& after_last_reads,
+                                               const float2* twl = nullptr) {
     using F = SwapFft<N>;
     const int t = threadIdx.x % F::TPA;
+    static_assert(!TWLS || AS == 1, "LDS twiddles of the swap schedule: one A-scan per lane");
+    // the twiddles of passes 1 and 2: registers, or the workgroup's table
+    auto tw1 = [&](int e) -> cf {
+        if constexpr (TWLS) {
+            const float2 w = twl[(e - 1) * 64 + 8 * F::d2_of(t) + F::r2_of(t)];
+            return {w.x, w.y};
+        } else {
+            return tw[1][e];
+        }
+    };
+    auto tw2 = [&](int e) -> cf {
+        if constexpr (TWLS) {
+            const float2 w = twl[448 + (e - F::RQ) * 8 + F::r2_of(t)];
+            return {w.x, w.y};
+        } else {
+            return tw[2][e];
+        }
+    };
     // pass 0 and exchange 1
     const int w1 = F::e1_write_base(t);
 #pragma unroll
@@ -811,7 +858,7 @@ __device__ __forceinline__ void fft_swap_small(cf (&v)[AS][8], float2* const (&p
     for (int u = 0; u < AS; ++u) {
         idft8(v[u]);
 #pragma unroll
-        for (int e = 1; e < 8; ++e) v[u][e] = cmul(v[u][e], tw[1][e]);
+        for (int e = 1; e < 8; ++e) v[u][e] = cmul(v[u][e], tw1(e));
 #pragma unroll
         for (int e = 0; e < 4; ++e) swap_lane_bit5(v[u][e], v[u][e + 4]);
         if constexpr (F::S == 4) {
@@ -828,7 +875,7 @@ __device__ __forceinline__ void fft_swap_small(cf (&v)[AS][8], float2* const (&p
             idft2(v[u][3], v[u][7]);
         }
 #pragma unroll
-        for (int e = F::RQ; e < 8; ++e) v[u][e] = cmul(v[u][e], tw[2][e]);
+        for (int e = F::RQ; e < 8; ++e) v[u][e] = cmul(v[u][e], tw2(e));
     }
     // exchange 3 and pass 3
     const int w3 = F::e3_write_base(t);
@@ -859,13 +906,14 @@ __device__ __forceinline__ void fft_swap_small(cf (&v)[AS][8], float2* const (&p
 // X[t + e*N/8] as after stockham_inverse.  Exchange 1 goes through pa,
 // exchange 3 through pb; ends without a barrier (other wavefronts may still
 // read pb).  Every lane of each wavefront must be active.
-template <int N, int AS, typename Hook = NoHook>
+template <int N, int AS, typename Hook = NoHook, bool TWLS = false>
 __device__ __forceinline__ void fft_swap(cf (&v)[AS][8], float2* const (&pa)[AS], float2* const (&pb)[AS],
-                                         const cf (&tw)[Geo<N>::NPASS][8], Hook&& after_last_reads = {}) {
+                                         const cf (&tw)[Geo<N>::NPASS][8], Hook&& after_last_reads = {},
+                                         const float2* twl = nullptr) {
     if constexpr (N == 4096) {
         fft_swap4096<AS>(v, pa, pb, tw, after_last_reads);
     } else {
-        fft_swap_small<N, AS>(v, pa, pb, tw, after_last_reads);
+        fft_swap_small<N, AS, Hook, TWLS>(v, pa, pb, tw, static_cast<Hook&&>(after_last_reads), twl);
     }
 }
 
@@ -1182,6 +1230,9 @@ struct ChainLds {
     // then fit four waves per SIMD; the DC prologue needs more registers)
     template <int T>
     static constexpr bool TWL = twl_fft<N, T, AS, SCHED>() && !PAIR && (MODE & (M_DC | M_STATS)) == 0;
+    // the swap schedule's passes 1 and 2 from LDS (twl_swap)
+    template <int T>
+    static constexpr bool TWLS = twl_swap<N, T, AS, SCHED>() && !PAIR && (MODE & M_STATS) == 0;
 };
 
 // The chain over A-scans a_begin + k*a_step ... (< a_end) of A, each
@@ -1265,7 +1316,11 @@ __device__ __forceinline__ void chain_body(const ChainArgs A, float2* bufA, floa
     }
     cf tw[NP][8];
     constexpr bool TWL = L::template TWL<T>;
+    constexpr bool TWLS = L::template TWLS<T>;
     if constexpr (SWAP) {
+        // (with the LDS table the registers of passes 1 and 2 go unread)
+        if constexpr (TWLS)
+            for (int i = threadIdx.x; i < kTwlEntries; i += WG) twl[i] = gload2(A.twiddle, twl_swap_exponent<N>(i));
         load_twiddles_swap<N>(A.twiddle, t, tw);
     } else if constexpr (TWL) {
         // passes 1 and 2 from the workgroup's LDS table, filled here; the
@@ -1579,7 +1634,7 @@ __device__ __forceinline__ void chain_body(const ChainArgs A, float2* bufA, floa
             }
         } else {
             if constexpr (SWAP)
-                fft_swap<N, AS>(v, pa, pb, tw, next);
+                fft_swap<N, AS, decltype(next)&, TWLS>(v, pa, pb, tw, next, twl);
             else
                 stockham_inverse<N, AS, SCHED, decltype(next)&, TWL>(v, pa, pb, tw, next, twl);
         }
@@ -1723,12 +1778,18 @@ __device__ __forceinline__ void chain_body(const ChainArgs A, float2* bufA, floa
 // spills; three is asked for wherever it needs no spills (every
 // throughput-mode kernel up to 4 taps, and DC removal of 8/16-bit samples).
 // Lanczos (T = 16) holds 128 tap weights per lane: two waves per SIMD with a
-// few spills measured faster than the one the compiler picks; Lanczos-4
-// (T = 8, 64 weights) does not fit three without spills either.  The FPN
-// statistics modes and the double-precision DC prologue keep the default.
+// few spills measured faster than the one the compiler picks.  Lanczos-4
+// (T = 8, 64 weights) fits three on the swap schedule with its LDS
+// twiddles (twl_swap; with DC removal of 8/16-bit samples 8-20 bytes of
+// spill per lane), two elsewhere.  The FPN statistics modes and the
+// double-precision DC prologue keep the default.
 template <int N, int T, int MODE, int IW, int AS, int SCHED>
 constexpr int chain_waves_per_eu() {
-    if (T >= 8) return 2;
+    if (T >= 8)
+        return (T == 8 && ChainLds<N, MODE, IW, AS, chain_wg<N>(), SCHED>::template TWLS<T> &&
+                !(IW == 4 && (MODE & M_DC)))
+                   ? 3
+                   : 2;
     if ((MODE & M_STATS) || (IW == 4 && (MODE & M_DC))) return OPE_WAVES_PER_EU;
     // N = 4096 with the LDS twiddles: two 512-lane workgroups per CU
     if (ChainLds<N, MODE, IW, AS, chain_wg<N>(), SCHED>::template TWL<T>) return 4;
@@ -1746,7 +1807,7 @@ chain_kernel(ChainArgs A) {
     __shared__ float xsep[L::X];
     __shared__ typename L::DcSum scan_part[L::SCAN];
     __shared__ float mm_part[L::MM];
-    __shared__ float2 twl[L::template TWL<T> ? kTwlEntries : 1];
+    __shared__ float2 twl[(L::template TWL<T> || L::template TWLS<T>) ? kTwlEntries : 1];
     constexpr int SLOTS = L::SLOTS;
     long long a_begin, a_step, a_end;
     if constexpr ((MODE & M_STATS) != 0) {
