@@ -630,9 +630,11 @@ struct SwapFft {
 // LDS (the same floats as the global table, so results are bit-identical)
 // and each lane reads its 7 + (8 - RQ) there next to the passes' data
 // reads.  That frees the 2 x 7 complex registers those passes held, and
-// the Lanczos-4 kernels fit three waves per SIMD instead of two (2048-point
-// frames 65.5 -> 63.4 us, with DC removal 95.5 -> 88.5, the 2048 x 1024 x
-// 256 volume 1.42-1.48 -> 1.33 ms).  Lanczos-8 stays at two waves with the
+// the Lanczos-4 kernels fit three waves per SIMD instead of two; with the
+// tap-window starts kept as 16-bit halves and the background constant read
+// in the epilogue, as for N = 4096 (2048-point frames 65.5 -> 63.0 us, with
+// DC removal 95.5 -> 85.4, the 2048 x 1024 x 256 volume 1.42-1.48 ->
+// 1.27-1.29 ms).  Lanczos-8 stays at two waves with the
 // registers freed and ran slower with the table (80.7 -> 87.6 us), so it
 // keeps its register twiddles.
 template <int N, int T, int AS, int SCHED>
@@ -1288,7 +1290,9 @@ __device__ __forceinline__ void chain_body(const ChainArgs A, float2* bufA, floa
     // other A-scans of the lane sit at constant offsets from it)
     // (with the LDS twiddles the window starts are kept as 16-bit halves,
     // two per register: 4 VGPRs fewer for one extraction per output)
-    constexpr bool PACK_TB = L::template TWL<T>;
+    // (the kernels with LDS twiddles, where registers are scarce)
+    constexpr bool LEAN = L::template TWL<T> || L::template TWLS<T>;
+    constexpr bool PACK_TB = LEAN;
     const float* tb[8];
     unsigned tbp[4];
     float tw8[8][T];
@@ -1346,7 +1350,7 @@ __device__ __forceinline__ void chain_body(const ChainArgs A, float2* bufA, floa
             } else {
                 fp[o] = {0.f, 0.f};
             }
-            bgv[o] = (!TWL && (flags & F_BG)) ? bg_const(k) : 0.f;
+            bgv[o] = (!LEAN && (flags & F_BG)) ? bg_const(k) : 0.f;
         }
     }
 
@@ -1724,7 +1728,7 @@ __device__ __forceinline__ void chain_body(const ChainArgs A, float2* bufA, floa
                     for (int o = 0; o < 4; ++o) {
                         // bgv is 0 without background subtraction: x - 0 == x exactly
                         float b = bgv[o];
-                        if constexpr (TWL) {
+                        if constexpr (LEAN) {
                             if (flags & F_BG) b = bg_const(t + o * TPA);
                         }
                         __builtin_nontemporal_store(res[o] - b, dst + o * TPA);

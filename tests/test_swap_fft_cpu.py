@@ -196,6 +196,51 @@ def test_planted_address_constant_conflicts():
     assert worst > 2
 
 
+
+# The Lanczos-4 kernels read the twiddles of passes 1 and 2 from a 504-entry
+# LDS table (chain_kernels.hpp twl_swap_exponent, fft_swap_small tw1/tw2).
+def twl_swap_exponent(n, i, rq):
+    if i < 448:
+        return (8 * (i % 64) * (1 + i // 64)) & (n - 1)
+    j = i - 448
+    return (64 * (j % 8) * ((rq + j // 8) // rq)) & (n - 1)
+
+
+def twl_swap_reads(F, t, plant=None):
+    """(entry index, register exponent) of every table read of lane t."""
+    r1 = 8 * F.d2_of(t) + F.r2_of(t)
+    out = []
+    for e in range(1, 8):
+        idx = (e - 1) * 64 + (F.r2_of(t) if plant == "r1" else r1)
+        out.append((idx, (8 * r1 * e) & (F.n - 1)))
+    for e in range(F.rq, 8):
+        idx = 448 + (e - F.rq) * 8 + F.r2_of(t)
+        out.append((idx, (64 * F.r2_of(t) * (e // F.rq)) & (F.n - 1)))
+    return out
+
+
+@pytest.mark.parametrize("n", [1024, 2048])
+def test_lds_twiddle_table_holds_the_register_twiddles(n):
+    F = Swap(n)
+    for t in range(F.tpa):
+        for idx, expo in twl_swap_reads(F, t):
+            assert 0 <= idx < 7 * 8 + 7 * 64
+            assert twl_swap_exponent(n, idx, F.rq) == expo, (t, idx)
+    # within a wavefront the reads of one entry row are broadcasts: at most
+    # 8 distinct entries per read instruction
+    for wave in range(F.tpa // 64):
+        for k in range(7 + 8 - F.rq):
+            idxs = {twl_swap_reads(F, t)[k][0] for t in range(64 * wave, 64 * wave + 64)}
+            assert len(idxs) <= 8
+
+
+def test_planted_lds_twiddle_row_mistake_is_caught():
+    F = Swap(2048)
+    bad = sum(twl_swap_exponent(2048, idx, F.rq) != expo
+              for t in range(F.tpa) for idx, expo in twl_swap_reads(F, t, plant="r1"))
+    assert bad > 0
+
+
 # ------------------------------------------------------------------ N = 4096
 #
 # Radices 8, 8, 8, 8 with n = 512 d0 + 64 d1 + 8 d2 + r2 and
