@@ -625,7 +625,9 @@ struct SwapFft {
 };
 
 // The swap schedule at N = 1024 and 2048 with 8 taps (Lanczos-4), one A-scan
-// per lane: the twiddles of its passes 1 and 2 depend on r1 = 8 d2 + r2
+// per lane (and cubic with DC removal of 8/16-bit samples, where the freed
+// registers take the banded DC path): the twiddles of its passes 1 and 2
+// depend on r1 = 8 d2 + r2
 // (64 values) and r2 (8 values) only, so the workgroup keeps them once in
 // LDS (the same floats as the global table, so results are bit-identical)
 // and each lane reads its 7 + (8 - RQ) there next to the passes' data
@@ -1234,7 +1236,9 @@ struct ChainLds {
     static constexpr bool TWL = twl_fft<N, T, AS, SCHED>() && !PAIR && (MODE & (M_DC | M_STATS)) == 0;
     // the swap schedule's passes 1 and 2 from LDS (twl_swap)
     template <int T>
-    static constexpr bool TWLS = twl_swap<N, T, AS, SCHED>() && !PAIR && (MODE & M_STATS) == 0;
+    static constexpr bool TWLS = (twl_swap<N, T, AS, SCHED>() || (T == 4 && twl_swap<N, 8, AS, SCHED>() &&
+                                                                    (MODE & M_DC) && IW <= 2)) &&
+                                 !PAIR && (MODE & M_STATS) == 0;
 };
 
 // The chain over A-scans a_begin + k*a_step ... (< a_end) of A, each
@@ -1456,13 +1460,16 @@ __device__ __forceinline__ void chain_body(const ChainArgs A, float2* bufA, floa
     // after the FFT uses the FFT buffers.)
     // The banded DC path (windows of 1 .. DC_BAND samples) needs ~29 more
     // registers at the staging point: the kernels with up to 2 taps take it
-    // (linear spills 8 bytes per lane at N = 2048 and still gains 9 %),
-    // cubic and Lanczos-8 would spill 50-100 and run slower.  Staged
+    // (linear spills 8 bytes per lane at N = 2048 and still gains 9 %), and
+    // cubic where the swap schedule's LDS twiddles free the registers
+    // (N = 1024 and 2048, 8/16-bit samples: 2048-point cubic + DC 75.6 ->
+    // 73.5 us); with register twiddles cubic and Lanczos-8 would spill
+    // 50-100 bytes and run slower.  Staged
     // inside the FFT, it splits: the prefix sums go in there, the window
     // means after the output stores, where the FFT's registers are free (the
     // raw words of the staged group stay in registers in between).  The
     // general path stages whole inside the FFT.
-    constexpr bool BANDED = T <= 2;
+    constexpr bool BANDED = T <= 2 || (T == 4 && L::template TWLS<T>);
     constexpr bool SPLIT = DC && BANDED && dc_banded<N, TPA, DcSum>();
     constexpr bool PIPE = DC ? L::DC_PIPE : !L::SWAP_TOP_BARRIER;
     auto stage_prefix = [&]() {
