@@ -29,7 +29,7 @@ def _chain(n, a, method, dc=False, window=32):
 # every variant: two A-scans per lane with the staging sharing buffer B,
 # the Stockham placements, and N = 4096 (512 lanes per A-scan)
 @pytest.mark.parametrize("n", [256, 1024, 2048, 4096])
-@pytest.mark.parametrize("method,dc", [(O.CUBIC, False), (O.LANCZOS, True), (O.LINEAR, True)])
+@pytest.mark.parametrize("method,dc", [(O.CUBIC, False), (O.CUBIC, True), (O.LANCZOS, True), (O.LINEAR, True)])
 def test_every_launch_variant_meets_parity(n, method, dc):
     a = 96
     p, t = _chain(n, a, method, dc, window=64 if method == O.LINEAR else 32)

@@ -122,13 +122,15 @@ def test_dc_removal(dtype, window):
     assert e <= DB_TOL
 
 
-# The banded DC path: kernels with up to 2 resampling taps (none, linear),
-# 8/16-bit samples, N >= 1024 and windows of 1 .. 64 samples.  Every window residue mod 8
+# The banded DC path: kernels with up to 2 resampling taps (none, linear)
+# and cubic on the swap schedule at N = 1024 and 2048 (at 4096 cubic takes
+# the general path), 8/16-bit samples, N >= 1024 and windows of 1 .. 64
+# samples.  Every window residue mod 8
 # (each a different placement of the padded prefix blocks), the widest band
 # windows, and 0 and 65 on the general path; the clipped windows at both
 # ends of every A-scan are part of each case.  Peak-normalised linear error
 # and dB error near the peak against the FP64 oracle.
-@pytest.mark.parametrize("method", ["none", "linear"])
+@pytest.mark.parametrize("method", ["none", "linear", "cubic"])
 @pytest.mark.parametrize("dtype", [np.uint8, np.uint16])
 @pytest.mark.parametrize("window", [0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 31, 56, 63, 64, 65])
 @pytest.mark.parametrize("n", [1024, 2048, 4096])
