@@ -1272,6 +1272,28 @@ __device__ __forceinline__ void chain_body(const ChainArgs A, float2* bufA, floa
     // one A-scan per workgroup iteration: the A-scan index is uniform
     const int slot = (WG == TPA) ? 0 : (int)threadIdx.x / TPA;
     const int t = threadIdx.x - slot * TPA;
+    // Row map on a dynamic schedule: the workgroup's first three chunks are
+    // taken before anything else (the ticket passes from lane 0 through the
+    // min/max partials' first slot), and a workgroup that finds none (the
+    // schedule's other launch took them all) counts itself out at once
+    // instead of reading per-lane tables it would not use.
+    unsigned first_ticket = 0;
+    if constexpr ((MODE & M_ROWS) != 0) {
+        if (A.tickets != nullptr) {
+            unsigned* const sh = reinterpret_cast<unsigned*>(mm_part);
+            if (threadIdx.x == 0) *sh = atomicAdd(A.tickets, 3u);
+            __syncthreads();
+            first_ticket = *sh;
+            __syncthreads();
+            if ((long long)first_ticket * kRowChunk * ChainLds<N, MODE, IW, AS, WG, SCHED>::SLOTS >= a_end) {
+                if (threadIdx.x == 0 && atomicAdd(A.tickets + 1, 1u) == A.ticket_wgs - 1) {
+                    atomicExch(A.tickets, 0u);
+                    atomicExch(A.tickets + 1, 0u);
+                }
+                return;
+            }
+        }
+    }
     float2* pa[AS];
     float2* pb[AS];
     float* xs[AS];
@@ -1405,12 +1427,9 @@ __device__ __forceinline__ void chain_body(const ChainArgs A, float2* bufA, floa
     unsigned* const tk_sh = reinterpret_cast<unsigned*>(mm_part);
     if constexpr (rowmap) {
         if (dyn) {
-            if (threadIdx.x == 0) *tk_sh = atomicAdd(A.tickets, 3u);
-            __syncthreads();
-            cur = *tk_sh;
+            cur = first_ticket;
             nxt = cur + 1;
             nn = cur + 2;
-            __syncthreads();
             a_begin = cur * CG;
             a_step = L::SLOTS;
         }
