@@ -627,18 +627,17 @@ struct SwapFft {
 // The swap schedule at N = 1024 and 2048 with 8 taps (Lanczos-4), one A-scan
 // per lane (and cubic with DC removal of 8/16-bit samples, where the freed
 // registers take the banded DC path): the twiddles of its passes 1 and 2
-// depend on r1 = 8 d2 + r2
-// (64 values) and r2 (8 values) only, so the workgroup keeps them once in
-// LDS (the same floats as the global table, so results are bit-identical)
-// and each lane reads its 7 + (8 - RQ) there next to the passes' data
-// reads.  That frees the 2 x 7 complex registers those passes held, and
-// the Lanczos-4 kernels fit three waves per SIMD instead of two; with the
-// tap-window starts kept as 16-bit halves and the background constant read
-// in the epilogue, as for N = 4096 (2048-point frames 65.5 -> 63.0 us, with
-// DC removal 95.5 -> 85.4, the 2048 x 1024 x 256 volume 1.42-1.48 ->
-// 1.27-1.29 ms).  Lanczos-8 stays at two waves with the
-// registers freed and ran slower with the table (80.7 -> 87.6 us), so it
-// keeps its register twiddles.
+// depend on r1 = 8 d2 + r2 (64 values) and r2 (8 values) only, so the
+// workgroup keeps them once in LDS (the same floats as the global table, so
+// results are bit-identical) and each lane reads its 7 + (8 - RQ) there
+// next to the passes' data reads.  That frees the 2 x 7 complex registers
+// those passes held: with the tap-window starts kept as 16-bit halves and
+// the background constant read in the epilogue, as for N = 4096, the
+// Lanczos-4 kernels fit three waves per SIMD instead of two (2048-point
+// frames 65.5 -> 63.0 us, with DC removal 95.5 -> 85.4, the 2048 x 1024 x
+// 256 volume 1.42-1.48 -> 1.27-1.30 ms).  Lanczos-8 stays at two waves with
+// the registers freed and ran slower with the table (80.7 -> 87.6 us), so
+// it keeps its register twiddles.
 template <int N, int T, int AS, int SCHED>
 constexpr bool twl_swap() {
     return (N == 1024 || N == 2048) && SCHED == Geo<N>::NPASS && AS == 1 && T == 8;
