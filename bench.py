@@ -427,14 +427,16 @@ SWEEP_METHODS = {"none": -1, "linear": 0, "cubic": 1, "lanczos": 2, "lanczos4": 
 def sweep_configs():
     """The chain's configurations timed by --sweep: every length with each
     k-linearization (dispersion on), linear and cubic without dispersion
-    (two real A-scans per complex FFT), DC removal (with the banded path
-    where there are no taps), and the 256/512-point
-    lengths with cubic.  Frames of 2^25 samples (16384
-    A-scans at N = 2048, as the benchmark workload)."""
+    (two real A-scans per complex FFT), DC removal (cubic at every length
+    from 1024 as the reference's benchmark runs it, Lanczos, and without
+    taps on the banded path), and the 256/512-point lengths with cubic.
+    Frames of 2^25 samples (16384 A-scans at N = 2048, as the benchmark
+    workload)."""
     cfgs = [(n, m, True, False) for n in (1024, 2048, 4096) for m in ("none", "linear", "cubic", "lanczos")]
     cfgs += [(n, m, False, False) for n in (1024, 2048, 4096) for m in ("linear", "cubic")]
     cfgs += [(n, "lanczos4", True, dc) for n in (1024, 2048, 4096) for dc in (False, True)]
-    cfgs += [(2048, "cubic", True, True), (2048, "lanczos", True, True),
+    cfgs += [(n, "cubic", True, True) for n in (1024, 2048, 4096)]
+    cfgs += [(2048, "lanczos", True, True),
              (1024, "none", True, True), (2048, "none", True, True),
              (256, "cubic", True, False), (512, "cubic", True, False)]
     return cfgs
